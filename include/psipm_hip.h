@@ -1,0 +1,141 @@
+/*
+ * psipm_hip.h -- C-ABI of libpsipm_hip.so, the MI355X (gfx950) hot path of
+ * PS-IPM: the preconditioned-CG solve of the proximal normal equations
+ *
+ *     (A W A^T + delta I) dy = rhs,      W = diag(1 / theta),
+ *
+ * preconditioned by the sparse Cholesky factor of the active-column matrix
+ *
+ *     M = A_C W_C A_C^T + diag(max(d_full - d_core, 0)) + delta I.
+ *
+ * Conventions
+ *   - every call returns int: 0 = success, < 0 = error (message from
+ *     psg_last_error()), > 0 = a numerical outcome documented per call;
+ *   - no C++ exception crosses this boundary;
+ *   - pointer arguments are caller-owned host buffers, borrowed for the call
+ *     only; outputs are written into caller-provided arrays;
+ *   - indices are int32 (column pointers int64), values IEEE float64;
+ *   - a context owns one LP on one GPU and is not reentrant.
+ *
+ * Each entry point names the reference code it replaces (files of
+ * StefanoCipolla/MF-PSIPM; PS = psipm_active_pool.py,
+ * SAP = psipm_stable_preconditioner.py, SSP = psipm_spectral_preconditioner.py).
+ */
+#ifndef PSIPM_HIP_H
+#define PSIPM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PSG_OK 0
+#define PSG_ERR_ARGUMENT (-1)
+#define PSG_ERR_DEVICE (-2)
+#define PSG_ERR_MEMORY (-3)
+#define PSG_ERR_STATE (-4)
+#define PSG_ERR_INTERNAL (-5)
+
+/* precision modes */
+#define PSG_FP64 0  /* everything FP64 */
+#define PSG_MIXED 1 /* FP32 SpMV values inside PCG, FP32 factor and trisolves;
+                       FP64 outer residuals, dots, step lengths */
+
+typedef struct psg_plan psg_plan; /* host symbolic analysis */
+typedef struct psg_ctx psg_ctx;   /* device state of one LP */
+
+/* PCG outcome, field-for-field the KrylovResult of SSP:105-116 (the residual
+ * vector stays on the device, see psg_pcg_residual). */
+typedef struct psg_pcg_result {
+  int32_t info;           /* 0 converged; >0 iteration limit; -1..-7 as SSP:486-553 */
+  int32_t iterations;
+  double relative_residual;
+  int32_t replacements;
+  int32_t residual_checks;
+  int32_t stagnated;
+  int32_t n_alpha;        /* Lanczos coefficients since the last replacement */
+  int32_t n_beta;
+  int32_t n_history;      /* explicit residual history length */
+} psg_pcg_result;
+
+/* ---- library ------------------------------------------------------------ */
+const char* psg_last_error(void);
+int psg_version(void);
+/* number of visible GPUs (0 on a host without one; never an error) */
+int psg_device_count(int32_t* count);
+
+/* ---- host symbolic analysis ----------------------------------------------
+ * Replaces CHOLMOD analyze (PS:668-671, order "best") and the pattern test of
+ * PS:842-846.  Pattern of M = union of the cliques of the active columns
+ * `core` of A (CSC, m x n, sorted, no duplicates).  ordering: 0 natural,
+ * 1 minimum degree.  max_width caps supernode width. */
+int psg_plan_create(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowind,
+                    const int32_t* core, int64_t ncore, int32_t ordering,
+                    int32_t max_width, double dense_factor, psg_plan** out);
+/* info[0..11] = m, nsuper, nlevels, nnz_panels, nnz_L, npairs, nsegments,
+ * max_width, max_rows, factor_flops, row_structure_total, 0 */
+int psg_plan_info(const psg_plan* plan, double* info);
+/* copy plan arrays out (any pointer may be NULL) -- used by host-side tests */
+int psg_plan_export(const psg_plan* plan, int32_t* perm, int32_t* sfirst, int64_t* rptr,
+                    int32_t* rows, int64_t* poff, int32_t* sparent, int32_t* level_ptr,
+                    int32_t* level_nodes, int64_t* diag_index, int64_t* seg_target,
+                    int64_t* seg_ptr, int32_t* pair_col, int64_t* pair_a, int64_t* pair_b);
+int psg_plan_destroy(psg_plan* plan);
+
+/* ---- context -------------------------------------------------------------
+ * Uploads A (CSC) and builds its CSR copy (NormalEquations.__init__,
+ * PS:781-784 and the At copy of PS:784). */
+int psg_create(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowind,
+               const double* values, int32_t precision, psg_ctx** out);
+int psg_destroy(psg_ctx* ctx);
+
+/* W = 1/theta and delta of the current Newton matrix (SAP:399, SSP:462-463) */
+int psg_set_weights(psg_ctx* ctx, const double* theta, double delta);
+
+/* out = A (W (A^T v)) + delta v  -- the exact operator (SAP:434-438) */
+int psg_apply_H(psg_ctx* ctx, const double* v, double* out);
+
+/* d_full = (A o A) w over all columns and d_core over the columns flagged in
+ * core_mask (SAP:379-390); either output may be NULL */
+int psg_diag_terms(psg_ctx* ctx, const uint8_t* core_mask, double* d_full, double* d_core);
+
+/* leverage scores of the candidate columns (SAP:293-309):
+ * s_j = w_j sum_i a_ij^2 / (d_full_i + max(delta, tiny)) / (1 + fill_weight q_j(q_j-1)/2) */
+int psg_scores(psg_ctx* ctx, const int32_t* candidates, int64_t ncand, double delta,
+               double fill_weight, double* scores);
+
+/* ---- preconditioner --------------------------------------------------------
+ * Assemble M for the active columns of `plan` with the current weights, scale
+ * it to unit diagonal (PS:850-853), add eta I and factor it on the GPU
+ * (PS:863-885 numeric step).  Returns 0 on success, or 1 + the permuted
+ * column of the first non-positive pivot (the caller climbs the shift ladder).
+ * The plan is uploaded on first use and cached while it stays the same. */
+int psg_factor(psg_ctx* ctx, const psg_plan* plan, double eta, double* nnz_factor);
+
+/* z = D P^T L^-T L^-1 P D r  -- _apply_base (PS:903-904) */
+int psg_apply_P(psg_ctx* ctx, const double* r, double* z);
+
+/* ---- solve ------------------------------------------------------------------
+ * rhs = r2 - A (W r1)  kept on the device (SSP:654-655) */
+int psg_solve_begin(psg_ctx* ctx, const double* r1, const double* r2);
+/* PCG on the stored rhs (SSP:469-581): x0 = P^-1 rhs, residual replacement
+ * every `replacement` iterations, gap test, info codes -1..-7.  alphas/betas
+ * (capacity maxit) and history (capacity maxit + 2) receive the Lanczos
+ * coefficients and the explicit residual history; any may be NULL. */
+int psg_pcg(psg_ctx* ctx, double tol, int32_t maxit, int32_t replacement, double gap_tol,
+            psg_pcg_result* result, double* alphas, double* betas, double* history);
+/* dy = last PCG solution, dx = W (r1 + A^T dy)  (SSP:715-716) */
+int psg_solve_finish(psg_ctx* ctx, double* dx, double* dy);
+/* final explicit residual rhs - H dy of the last PCG run (KrylovResult.residual) */
+int psg_pcg_residual(psg_ctx* ctx, double* residual);
+
+/* ---- timing ---------------------------------------------------------------- */
+/* device time (ms, HIP events) of the last psg_pcg and last psg_factor calls */
+int psg_last_timings(psg_ctx* ctx, double* pcg_ms, double* factor_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSIPM_HIP_H */

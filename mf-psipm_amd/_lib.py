@@ -1,0 +1,124 @@
+"""ctypes binding of libpsipm_hip.so (the C-ABI declared in include/psipm_hip.h).
+
+The library is built in-tree by ``build.py`` (hipcc, gfx950).  Loading fails
+loudly when it is missing: there is no CPU fallback for the hot path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_NAME = "libpsipm_hip.so"
+LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
+
+PSG_OK = 0
+PSG_FP64 = 0
+PSG_MIXED = 1
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+class PsgPcgResult(C.Structure):
+    _fields_ = [
+        ("info", C.c_int32),
+        ("iterations", C.c_int32),
+        ("relative_residual", C.c_double),
+        ("replacements", C.c_int32),
+        ("residual_checks", C.c_int32),
+        ("stagnated", C.c_int32),
+        ("n_alpha", C.c_int32),
+        ("n_beta", C.c_int32),
+        ("n_history", C.c_int32),
+    ]
+
+
+class LibraryError(RuntimeError):
+    """A C-ABI call returned a negative status."""
+
+    def __init__(self, name: str, status: int, message: str):
+        super().__init__(f"{name} failed with status {status}: {message}")
+        self.status = status
+
+
+def _opt(ptype):
+    """Pointer argument that also accepts None (passed as NULL)."""
+
+    class _Nullable:
+        @classmethod
+        def from_param(cls, obj):
+            if obj is None:
+                return None
+            return ptype.from_param(obj)
+
+    return _Nullable
+
+
+# (name, restype, argtypes) of every exported symbol in include/psipm_hip.h
+SIGNATURES = [
+    ("psg_last_error", C.c_char_p, []),
+    ("psg_version", C.c_int, []),
+    ("psg_device_count", C.c_int, [C.POINTER(C.c_int32)]),
+    ("psg_plan_create", C.c_int, [C.c_int64, C.c_int64, _i64p, _i32p, _i32p, C.c_int64, C.c_int32,
+                                  C.c_int32, C.c_double, C.POINTER(C.c_void_p)]),
+    ("psg_plan_info", C.c_int, [C.c_void_p, _f64p]),
+    ("psg_plan_export", C.c_int, [C.c_void_p] + [
+        _opt(t) for t in (_i32p, _i32p, _i64p, _i32p, _i64p, _i32p, _i32p, _i32p, _i64p, _i64p,
+                          _i64p, _i32p, _i64p, _i64p)]),
+    ("psg_plan_destroy", C.c_int, [C.c_void_p]),
+    ("psg_create", C.c_int, [C.c_int64, C.c_int64, _i64p, _i32p, _f64p, C.c_int32,
+                             C.POINTER(C.c_void_p)]),
+    ("psg_destroy", C.c_int, [C.c_void_p]),
+    ("psg_set_weights", C.c_int, [C.c_void_p, _f64p, C.c_double]),
+    ("psg_apply_H", C.c_int, [C.c_void_p, _f64p, _f64p]),
+    ("psg_diag_terms", C.c_int, [C.c_void_p, _opt(_u8p), _opt(_f64p), _opt(_f64p)]),
+    ("psg_scores", C.c_int, [C.c_void_p, _i32p, C.c_int64, C.c_double, C.c_double, _f64p]),
+    ("psg_factor", C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double)]),
+    ("psg_apply_P", C.c_int, [C.c_void_p, _f64p, _f64p]),
+    ("psg_solve_begin", C.c_int, [C.c_void_p, _f64p, _f64p]),
+    ("psg_pcg", C.c_int, [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_double,
+                          C.POINTER(PsgPcgResult), _opt(_f64p), _opt(_f64p), _opt(_f64p)]),
+    ("psg_solve_finish", C.c_int, [C.c_void_p, _f64p, _f64p]),
+    ("psg_pcg_residual", C.c_int, [C.c_void_p, _f64p]),
+    ("psg_last_timings", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+]
+
+_lib = None
+
+
+def load(path: os.PathLike | str | None = None):
+    """Load the library once and attach the C signatures."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise ImportError(
+            f"{p} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    lib = C.CDLL(str(p))
+    for name, restype, argtypes in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(name: str, status: int) -> int:
+    """Raise LibraryError for negative status codes; return the status otherwise."""
+    if status < 0:
+        msg = load().psg_last_error()
+        raise LibraryError(name, status, msg.decode() if msg else "")
+    return status
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    check("psg_device_count", load().psg_device_count(C.byref(n)))
+    return int(n.value)

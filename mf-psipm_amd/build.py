@@ -1,0 +1,68 @@
+"""Build libpsipm_hip.so in-tree with hipcc for gfx950.
+
+Usage: python mf-psipm_amd/build.py [--jobs N]
+Compiles every csrc/*.hip (device + host) and csrc/*.cpp (host only) and
+links one shared library next to this file.  Objects go to csrc/_build/.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+OUT = HERE / "libpsipm_hip.so"
+ARCH = os.environ.get("PSG_OFFLOAD_ARCH", "gfx950")
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{HERE.parent / 'include'}"]
+HIP_FLAGS = [f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
+             # packed FP32 instructions stay off in device code
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def compile_one(src: Path, obj: Path) -> None:
+    cmd = [hipcc(), "-c"] + ([] if src.suffix == ".hip" else ["-x", "c++"]) + [str(src), "-o", str(obj)] + COMMON
+    if src.suffix == ".hip":
+        cmd += HIP_FLAGS
+    else:
+        pass
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    noise = [ln for ln in res.stderr.splitlines() if "packed-fp32-ops" not in ln and ln.strip()]
+    if noise:
+        print("\n".join(noise), file=sys.stderr)
+
+
+def build(jobs: int = 4) -> Path:
+    objdir = CSRC / "_build"
+    objdir.mkdir(exist_ok=True)
+    sources = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
+    objs = [objdir / (s.name + ".o") for s in sources]
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda so: compile_one(*so), zip(sources, objs)))
+    tmp = OUT.with_suffix(".so.tmp")
+    cmd = [hipcc(), "-shared", "-o", str(tmp)] + [str(o) for o in objs] + [f"--offload-arch={ARCH}", "-fPIC"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=4)
+    print(build(ap.parse_args().jobs))

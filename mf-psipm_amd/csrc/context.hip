@@ -1,0 +1,688 @@
+// Context, SpMV / BLAS-1 kernels and the PCG driver of libpsipm_hip.
+//
+// Hot loop of the reference (psipm_spectral_preconditioner.py:469-581,
+// SSP `_pcg`), re-designed for one MI355X:
+//   * H p = A (W (A^T p)) + delta p runs as two gather kernels -- a CSC
+//     column gather fused with the W scaling (no atomics) and a CSR row
+//     gather fused with the delta term and the three dot products PCG needs
+//     next (p.q, p.p, q.q).  Each segment (column or row) is owned by a group
+//     of G lanes that stream its nonzeros contiguously (coalesced HBM reads);
+//     the group reduces with cross-lane shuffles.
+//   * every reduction is two-stage and deterministic: blocks write partials,
+//     consumer kernels re-reduce them in a fixed order, so step lengths are
+//     computed on the device and the host synchronises once per iteration.
+//   * all scalars, dot products, residuals and step lengths are FP64.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "ctx.hpp"
+
+namespace psg {
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+// out_j = scale_j * (sum_p val[p] v[idx[p]] + add_j)   over segments j
+// (CSC: segments are columns -> W o (A^T v); with add = r1 it is dx of SSP:716)
+template <int G, class TV>
+__global__ void __launch_bounds__(kBlock)
+k_seg_gather(int nseg, const int* __restrict__ ptr, const int* __restrict__ idx,
+             const TV* __restrict__ val, const double* __restrict__ v,
+             const double* __restrict__ scale, const double* __restrict__ add,
+             double* __restrict__ out) {
+  const int lane = threadIdx.x % G;
+  const int groups_per_block = kBlock / G;
+  for (int64_t seg = (int64_t)blockIdx.x * groups_per_block + threadIdx.x / G; seg < nseg;
+       seg += (int64_t)gridDim.x * groups_per_block) {
+    const int b = ptr[seg], e = ptr[seg + 1];
+    double s = 0.0;
+    for (int p = b + lane; p < e; p += G) s += (double)val[p] * v[idx[p]];
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) {
+      double a = add ? add[seg] : 0.0;
+      out[seg] = (scale ? scale[seg] : 1.0) * (s + a);
+    }
+  }
+}
+
+enum CsrMode { kModeHv = 0, kModeResidual = 1, kModeRhs = 2 };
+
+// Row gather y_i = sum_p val[p] t[idx[p]] with a fused epilogue:
+//   kModeHv       : y = sum + delta v_i          partials (v.y, v.v, y.y)
+//   kModeResidual : y = b_i - (sum + delta v_i)  partials (y.y, |y - rref|^2)
+//   kModeRhs      : y = b_i - sum                partials (y.y)
+template <int G, class TV, int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_csr(int m, const int* __restrict__ ptr, const int* __restrict__ idx, const TV* __restrict__ val,
+      const double* __restrict__ t, const double* __restrict__ v, double delta,
+      const double* __restrict__ bvec, const double* __restrict__ rref, double* __restrict__ out,
+      double* __restrict__ part) {
+  const int lane = threadIdx.x % G;
+  const int groups_per_block = kBlock / G;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int64_t row = (int64_t)blockIdx.x * groups_per_block + threadIdx.x / G; row < m;
+       row += (int64_t)gridDim.x * groups_per_block) {
+    const int b = ptr[row], e = ptr[row + 1];
+    double s = 0.0;
+    for (int p = b + lane; p < e; p += G) s += (double)val[p] * t[idx[p]];
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) {
+      if (MODE == kModeHv) {
+        double vi = v[row];
+        double y = s + delta * vi;
+        out[row] = y;
+        acc[0] += vi * y;
+        acc[1] += vi * vi;
+        acc[2] += y * y;
+      } else if (MODE == kModeResidual) {
+        double y = bvec[row] - (s + delta * v[row]);
+        out[row] = y;
+        acc[0] += y * y;
+        if (rref) {
+          double d = y - rref[row];
+          acc[1] += d * d;
+        }
+      } else {
+        double y = bvec[row] - s;
+        out[row] = y;
+        acc[0] += y * y;
+      }
+    }
+  }
+  if (part) write_partials<3>(acc, part);
+}
+
+__global__ void k_recip(int n, const double* __restrict__ a, double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = 1.0 / a[i];
+}
+
+__global__ void k_mul(int n, const double* __restrict__ a, const double* __restrict__ b,
+                      double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = a[i] * b[i];
+}
+
+// partials of a.b (and b.b when want2)
+__global__ void __launch_bounds__(kBlock)
+k_dot(int n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ part) {
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double bi = b[i];
+    acc[0] += a[i] * bi;
+    acc[1] += bi * bi;
+  }
+  write_partials<2>(acc, part);
+}
+
+// alpha = rz / (p.q) with the curvature and step guards of SSP:490-501, then
+// x += alpha p, r -= alpha q and partials of r.r.  Every block reduces the
+// same partials, so every block computes the same alpha.
+__global__ void __launch_bounds__(kBlock)
+k_pcg_step(int m, const double* __restrict__ partA, int nbA, const double* __restrict__ rz_cur,
+           const double* __restrict__ p, const double* __restrict__ q, double* __restrict__ x,
+           double* __restrict__ r, double* __restrict__ scal, double* __restrict__ alpha_out) {
+  if (scal[kStop] != 0.0) return;
+  double tot[3];
+  reduce_partials<3>(partA, nbA, tot);
+  const double curvature = tot[0];
+  const double floor_ = std::numeric_limits<double>::min() * fmax(1.0, sqrt(tot[1]) * sqrt(tot[2]));
+  const double alpha = *rz_cur / curvature;
+  int fail = 0;
+  if (!isfinite(curvature) || curvature <= floor_) fail = -2;
+  else if (!isfinite(alpha) || alpha <= 0.0) fail = -3;
+  if (fail) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      scal[kInfo] = fail;
+      scal[kCurv] = curvature;
+    }
+    // kStop is raised by the host-visible copy below; other blocks of this
+    // launch read it only at entry, so raise it last.
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[kStop] = 1.0;
+    return;
+  }
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    acc[0] += ri * ri;
+  }
+  block_sum<1>(acc);
+  if (threadIdx.x == 0) {
+    scal[kScalarHeader + blockIdx.x] = acc[0];
+    if (blockIdx.x == 0) {
+      scal[kAlpha] = alpha;
+      scal[kCurv] = curvature;
+      *alpha_out = alpha;
+    }
+  }
+}
+
+// rz_new = r.z (from partials), guards of SSP:545-553, beta = rz_new / rz,
+// p = z + beta p.
+__global__ void __launch_bounds__(kBlock)
+k_pcg_direction(int m, const double* __restrict__ partC, int nbC, const double* __restrict__ rz_cur,
+                double* __restrict__ rz_next, const double* __restrict__ z, double* __restrict__ p,
+                double* __restrict__ scal, double* __restrict__ betas) {
+  if (scal[kStop] != 0.0) return;
+  double tot[2];
+  reduce_partials<2>(partC, nbC, tot);
+  const double rz_new = tot[0];
+  const double beta = rz_new / *rz_cur;
+  int fail = 0;
+  if (!isfinite(rz_new) || rz_new <= 0.0) fail = -6;
+  else if (!isfinite(beta) || beta < 0.0) fail = -7;
+  if (fail) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      scal[kInfo] = fail;
+      scal[kStop] = 1.0;
+    }
+    return;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *rz_next = rz_new;
+    const int nb = (int)scal[kNBeta];
+    betas[nb] = beta;
+    scal[kNBeta] = nb + 1;
+    scal[kBeta] = beta;
+  }
+}
+
+// d_full_i = sum_j a_ij^2 w_j [full_mask_j], d_core_i = same over core_mask (SAP:379-390)
+__global__ void __launch_bounds__(kBlock)
+k_diag_terms(int m, const int* __restrict__ ptr, const int* __restrict__ idx,
+             const double* __restrict__ val, const double* __restrict__ w,
+             const uint8_t* __restrict__ core_mask, double* __restrict__ full, double* __restrict__ core) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    double f = 0.0, c = 0.0;
+    for (int p = ptr[i]; p < ptr[i + 1]; ++p) {
+      const int j = idx[p];
+      const double a = val[p];
+      const double t = a * a * w[j];
+      f += t;
+      if (core_mask && core_mask[j]) c += t;
+    }
+    if (full) full[i] = f;
+    if (core) core[i] = c;
+  }
+}
+
+// leverage score of candidate columns (SAP:293-309)
+__global__ void __launch_bounds__(kBlock)
+k_scores(int ncand, const int* __restrict__ cand, const int* __restrict__ ptr,
+         const int* __restrict__ idx, const double* __restrict__ val, const double* __restrict__ w,
+         const double* __restrict__ d_full, double delta, double fill_weight,
+         double* __restrict__ out) {
+  const double tiny = std::numeric_limits<double>::min();
+  const double dd = fmax(delta, tiny);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < ncand; k += (int64_t)gridDim.x * blockDim.x) {
+    const int j = cand[k];
+    double s = 0.0;
+    for (int p = ptr[j]; p < ptr[j + 1]; ++p) {
+      const double a = val[p];
+      s += a * a * (1.0 / fmax(d_full[idx[p]] + dd, tiny));
+    }
+    const double q = (double)(ptr[j + 1] - ptr[j]);
+    out[k] = w[j] * s / (1.0 + fill_weight * 0.5 * q * fmax(q - 1.0, 0.0));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+static int pick_group(double avg) {
+  int g = 1;
+  while (g < 64 && g < avg) g <<= 1;
+  return g;
+}
+
+template <class TV>
+static void launch_gather(int G, int nseg, const int* ptr, const int* idx, const TV* val,
+                          const double* v, const double* scale, const double* add, double* out,
+                          hipStream_t s) {
+  const int grid = grid_for((int64_t)nseg * G, kBlock, 65535);
+  switch (G) {
+#define CASE(g) case g: hipLaunchKernelGGL((k_seg_gather<g, TV>), dim3(grid), dim3(kBlock), 0, s, nseg, ptr, idx, val, v, scale, add, out); break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
+#undef CASE
+    default: throw Error(PSG_ERR_INTERNAL, "bad group size");
+  }
+  HIP_LAUNCH_CHECK();
+}
+
+template <class TV, int MODE>
+static int launch_csr(int G, int m, const int* ptr, const int* idx, const TV* val, const double* t,
+                      const double* v, double delta, const double* b, const double* rref,
+                      double* out, double* part, hipStream_t s) {
+  const int grid = grid_for((int64_t)m * G, kBlock, kRedBlocks);
+  switch (G) {
+#define CASE(g) case g: hipLaunchKernelGGL((k_csr<g, TV, MODE>), dim3(grid), dim3(kBlock), 0, s, m, ptr, idx, val, t, v, delta, b, rref, out, part); break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
+#undef CASE
+    default: throw Error(PSG_ERR_INTERNAL, "bad group size");
+  }
+  HIP_LAUNCH_CHECK();
+  return grid;
+}
+
+// out = H v on device vectors; the FP32 copy of A is used only when lowp.
+// Returns the partials grid (partials in `part`: v.y, v.v, y.y).
+static int device_Hv(psg_ctx* c, const double* v, double* out, double* part, bool lowp) {
+  if (lowp && c->precision == PSG_MIXED) {
+    launch_gather<float>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval32.p, v, c->w.p, nullptr, c->t_n.p, c->stream);
+    return launch_csr<float, kModeHv>(c->g_csr, c->m, c->rptr.p, c->rcol.p, c->rval32.p, c->t_n.p, v,
+                                      c->delta, nullptr, nullptr, out, part, c->stream);
+  }
+  launch_gather<double>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval.p, v, c->w.p, nullptr, c->t_n.p, c->stream);
+  return launch_csr<double, kModeHv>(c->g_csr, c->m, c->rptr.p, c->rcol.p, c->rval.p, c->t_n.p, v,
+                                     c->delta, nullptr, nullptr, out, part, c->stream);
+}
+
+// out = b - H v (always FP64: the outer residual); partials (|out|^2, |out - rref|^2)
+static int device_residual(psg_ctx* c, const double* b, const double* v, const double* rref,
+                           double* out, double* part) {
+  launch_gather<double>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval.p, v, c->w.p, nullptr, c->t_n.p, c->stream);
+  return launch_csr<double, kModeResidual>(c->g_csr, c->m, c->rptr.p, c->rcol.p, c->rval.p, c->t_n.p, v,
+                                           c->delta, b, rref, out, part, c->stream);
+}
+
+static int device_dot(psg_ctx* c, const double* a, const double* b, int n, double* part) {
+  const int grid = grid_for(n, kBlock, kRedBlocks);
+  hipLaunchKernelGGL(k_dot, dim3(grid), dim3(kBlock), 0, c->stream, n, a, b, part);
+  HIP_LAUNCH_CHECK();
+  return grid;
+}
+
+static void read_partials(psg_ctx* c, const double* part, int nb, int np, double* out) {
+  std::vector<double> h((size_t)nb * np);
+  HIP_CHECK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < np; ++k) out[k] = host_sum(h.data() + (size_t)k * nb, nb);
+}
+
+static void copy_dev(psg_ctx* c, double* dst, const double* src, int n) {
+  HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+}
+
+void diag_terms_device(psg_ctx* c, const uint8_t* mask_dev, double* full, double* core) {
+  const int grid = grid_for(c->m, kBlock, 4096);
+  hipLaunchKernelGGL(k_diag_terms, dim3(grid), dim3(kBlock), 0, c->stream, c->m, c->rptr.p, c->rcol.p,
+                     c->rval.p, c->w.p, mask_dev, full, core);
+  HIP_LAUNCH_CHECK();
+}
+
+static void need_weights(const psg_ctx* c) {
+  if (!c->have_weights) throw Error(PSG_ERR_STATE, "weights not set (call psg_set_weights)");
+}
+
+}  // namespace psg
+
+using namespace psg;
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int psg_device_count(int32_t* count) {
+  return guarded([&]() -> int {
+    if (!count) throw Error(PSG_ERR_ARGUMENT, "null count");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return PSG_OK;
+  });
+}
+
+int psg_create(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowind,
+               const double* values, int32_t precision, psg_ctx** out) {
+  return guarded([&]() -> int {
+    if (!out || m <= 0 || n <= 0 || !colptr || !rowind || !values)
+      throw Error(PSG_ERR_ARGUMENT, "psg_create: invalid arguments");
+    if (precision != PSG_FP64 && precision != PSG_MIXED)
+      throw Error(PSG_ERR_ARGUMENT, "psg_create: unknown precision");
+    const int64_t nnz = colptr[n];
+    if (m >= (1LL << 31) || n >= (1LL << 31) || nnz >= (1LL << 31))
+      throw Error(PSG_ERR_ARGUMENT, "psg_create: sizes exceed int32 indexing");
+    if (colptr[0] != 0) throw Error(PSG_ERR_ARGUMENT, "psg_create: colptr[0] != 0");
+    for (int64_t j = 0; j < n; ++j)
+      if (colptr[j + 1] < colptr[j]) throw Error(PSG_ERR_ARGUMENT, "psg_create: colptr not monotone");
+    for (int64_t p = 0; p < nnz; ++p)
+      if (rowind[p] < 0 || rowind[p] >= m) throw Error(PSG_ERR_ARGUMENT, "psg_create: row index out of range");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+      throw Error(PSG_ERR_DEVICE, "psg_create: no GPU visible");
+    auto* c = new psg_ctx;
+    try {
+      c->m = (int)m;
+      c->n = (int)n;
+      c->nnz = nnz;
+      c->precision = precision;
+      HIP_CHECK(hipGetDevice(&c->device));
+      HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      HIP_CHECK(hipEventCreate(&c->ev0));
+      HIP_CHECK(hipEventCreate(&c->ev1));
+      // CSC (int32) and its transpose (CSR) built on the host, both uploaded
+      std::vector<int> cptr(n + 1), crow(rowind, rowind + nnz);
+      for (int64_t j = 0; j <= n; ++j) cptr[j] = (int)colptr[j];
+      std::vector<int> rptr(m + 1, 0), rcol(nnz);
+      std::vector<double> rval(nnz);
+      for (int64_t p = 0; p < nnz; ++p) rptr[rowind[p] + 1]++;
+      for (int64_t i = 0; i < m; ++i) rptr[i + 1] += rptr[i];
+      {
+        std::vector<int> fill(rptr.begin(), rptr.end() - 1);
+        for (int64_t j = 0; j < n; ++j)
+          for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) {
+            int q = fill[rowind[p]]++;
+            rcol[q] = (int)j;
+            rval[q] = values[p];
+          }
+      }
+      c->cptr.upload(cptr, c->stream);
+      c->crow.upload(crow, c->stream);
+      c->cval.upload(values, nnz, c->stream);
+      c->rptr.upload(rptr, c->stream);
+      c->rcol.upload(rcol, c->stream);
+      c->rval.upload(rval, c->stream);
+      if (precision == PSG_MIXED) {
+        std::vector<float> cv(values, values + nnz), rv(rval.begin(), rval.end());
+        c->cval32.upload(cv, c->stream);
+        c->rval32.upload(rv, c->stream);
+      }
+      c->g_csc = pick_group((double)nnz / (double)n);
+      c->g_csr = pick_group((double)nnz / (double)m);
+      c->theta.alloc(n);
+      c->w.alloc(n);
+      c->t_n.alloc(n);
+      c->r1_n.alloc(n);
+      c->dx_n.alloc(n);
+      for (auto* b : {&c->rhs, &c->x, &c->r, &c->z, &c->p, &c->q, &c->rt, &c->vin, &c->vout,
+                      &c->d_full, &c->d_core, &c->dsc})
+        b->alloc(m);
+      c->partA.alloc(3 * kRedBlocks);
+      c->partC.alloc(3 * kRedBlocks);
+      c->partD.alloc(3 * kRedBlocks);
+      c->scal.alloc(kScalarHeader + kRedBlocks);
+      HIP_CHECK(hipHostMalloc((void**)&c->hscal, (kScalarHeader + kRedBlocks) * sizeof(double)));
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+    } catch (...) {
+      psg_destroy(c);
+      throw;
+    }
+    *out = c;
+    return PSG_OK;
+  });
+}
+
+int psg_destroy(psg_ctx* c) {
+  if (!c) return PSG_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  factor_destroy(c);
+  if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PSG_OK;
+}
+
+int psg_set_weights(psg_ctx* c, const double* theta, double delta) {
+  return guarded([&]() -> int {
+    if (!c || !theta) throw Error(PSG_ERR_ARGUMENT, "psg_set_weights: null argument");
+    c->theta.upload(theta, c->n, c->stream);
+    hipLaunchKernelGGL(k_recip, dim3(grid_for(c->n, kBlock, 4096)), dim3(kBlock), 0, c->stream, c->n,
+                       c->theta.p, c->w.p);
+    HIP_LAUNCH_CHECK();
+    c->delta = delta;
+    c->have_weights = true;
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PSG_OK;
+  });
+}
+
+int psg_apply_H(psg_ctx* c, const double* v, double* out) {
+  return guarded([&]() -> int {
+    if (!c || !v || !out) throw Error(PSG_ERR_ARGUMENT, "psg_apply_H: null argument");
+    need_weights(c);
+    c->vin.upload(v, c->m, c->stream);
+    device_Hv(c, c->vin.p, c->vout.p, nullptr, false);
+    c->vout.download(out, c->m, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PSG_OK;
+  });
+}
+
+int psg_diag_terms(psg_ctx* c, const uint8_t* core_mask, double* d_full, double* d_core) {
+  return guarded([&]() -> int {
+    if (!c) throw Error(PSG_ERR_ARGUMENT, "psg_diag_terms: null context");
+    need_weights(c);
+    const uint8_t* mdev = nullptr;
+    if (core_mask) {
+      c->mask.upload(core_mask, c->n, c->stream);
+      mdev = c->mask.p;
+    }
+    diag_terms_device(c, mdev, c->d_full.p, c->d_core.p);
+    if (d_full) c->d_full.download(d_full, c->m, c->stream);
+    if (d_core) c->d_core.download(d_core, c->m, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PSG_OK;
+  });
+}
+
+int psg_scores(psg_ctx* c, const int32_t* candidates, int64_t ncand, double delta, double fill_weight,
+               double* scores) {
+  return guarded([&]() -> int {
+    if (!c || (ncand && (!candidates || !scores))) throw Error(PSG_ERR_ARGUMENT, "psg_scores: null argument");
+    need_weights(c);
+    if (ncand == 0) return PSG_OK;
+    for (int64_t k = 0; k < ncand; ++k)
+      if (candidates[k] < 0 || candidates[k] >= c->n) throw Error(PSG_ERR_ARGUMENT, "psg_scores: candidate out of range");
+    diag_terms_device(c, nullptr, c->d_full.p, nullptr);
+    c->cand.upload(candidates, ncand, c->stream);
+    c->cand_out.grow(ncand);
+    hipLaunchKernelGGL(k_scores, dim3(grid_for(ncand, kBlock, 8192)), dim3(kBlock), 0, c->stream, (int)ncand,
+                       c->cand.p, c->cptr.p, c->crow.p, c->cval.p, c->w.p, c->d_full.p, delta, fill_weight,
+                       c->cand_out.p);
+    HIP_LAUNCH_CHECK();
+    c->cand_out.download(scores, ncand, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PSG_OK;
+  });
+}
+
+int psg_apply_P(psg_ctx* c, const double* r, double* z) {
+  return guarded([&]() -> int {
+    if (!c || !r || !z) throw Error(PSG_ERR_ARGUMENT, "psg_apply_P: null argument");
+    if (!factor_ready(c)) throw Error(PSG_ERR_STATE, "psg_apply_P: no factor");
+    c->vin.upload(r, c->m, c->stream);
+    factor_apply(c, c->vin.p, c->vout.p);
+    c->vout.download(z, c->m, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PSG_OK;
+  });
+}
+
+int psg_solve_begin(psg_ctx* c, const double* r1, const double* r2) {
+  return guarded([&]() -> int {
+    if (!c || !r1 || !r2) throw Error(PSG_ERR_ARGUMENT, "psg_solve_begin: null argument");
+    need_weights(c);
+    c->r1_n.upload(r1, c->n, c->stream);
+    c->vin.upload(r2, c->m, c->stream);
+    // u = W r1 ; rhs = r2 - A u   (SSP:654-655)
+    hipLaunchKernelGGL(k_mul, dim3(grid_for(c->n, kBlock, 4096)), dim3(kBlock), 0, c->stream, c->n, c->w.p,
+                       c->r1_n.p, c->t_n.p);
+    HIP_LAUNCH_CHECK();
+    launch_csr<double, kModeRhs>(c->g_csr, c->m, c->rptr.p, c->rcol.p, c->rval.p, c->t_n.p, nullptr, 0.0,
+                                 c->vin.p, nullptr, c->rhs.p, nullptr, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->have_rhs = true;
+    c->have_solution = false;
+    return PSG_OK;
+  });
+}
+
+int psg_pcg(psg_ctx* c, double tol, int32_t maxit, int32_t replacement, double gap_tol,
+            psg_pcg_result* res, double* alphas_out, double* betas_out, double* history_out) {
+  return guarded([&]() -> int {
+    if (!c || !res || maxit < 1) throw Error(PSG_ERR_ARGUMENT, "psg_pcg: invalid arguments");
+    if (!c->have_rhs) throw Error(PSG_ERR_STATE, "psg_pcg: call psg_solve_begin first");
+    if (!factor_ready(c)) throw Error(PSG_ERR_STATE, "psg_pcg: no factor");
+    const int m = c->m;
+    hipStream_t s = c->stream;
+    c->rzs.grow(maxit + 2);
+    c->alphas.grow(maxit + 2);
+    c->betas.grow(maxit + 2);
+    HIP_CHECK(hipEventRecord(c->ev0, s));
+    HIP_CHECK(hipMemsetAsync(c->scal.p, 0, kScalarHeader * sizeof(double), s));
+    double tmp[3];
+    int nb = device_dot(c, c->rhs.p, c->rhs.p, m, c->partD.p);
+    read_partials(c, c->partD.p, nb, 1, tmp);
+    const double scale = std::max(1.0, std::sqrt(tmp[0]));
+    // x0 = P^-1 rhs ; r = rhs - H x0 ; z = P^-1 r ; rz = r.z ; p = z
+    factor_apply(c, c->rhs.p, c->x.p);
+    nb = device_residual(c, c->rhs.p, c->x.p, nullptr, c->r.p, c->partD.p);
+    read_partials(c, c->partD.p, nb, 1, tmp);
+    std::vector<double> hist;
+    hist.push_back(std::sqrt(tmp[0]) / scale);
+    factor_apply(c, c->r.p, c->z.p);
+    nb = device_dot(c, c->r.p, c->z.p, m, c->partD.p);
+    read_partials(c, c->partD.p, nb, 1, tmp);
+    double rz = tmp[0];
+    HIP_CHECK(hipMemcpyAsync(c->rzs.p, &rz, sizeof(double), hipMemcpyHostToDevice, s));
+    copy_dev(c, c->p.p, c->z.p, m);
+    int info = maxit;
+    if (hist.back() <= tol) info = 0;
+    else if (!std::isfinite(rz) || rz <= 0.0) info = -1;
+    int it = 0, rz_slot = 0, alpha_start = 0, beta_start = 0;
+    int replacements = 0, checks = 0;
+    const int grid_m = grid_for(m, kBlock, kRedBlocks);
+    while (info > 0 && it < maxit) {
+      const int nbA = device_Hv(c, c->p.p, c->q.p, c->partA.p, true);
+      hipLaunchKernelGGL(k_pcg_step, dim3(grid_m), dim3(kBlock), 0, s, m, c->partA.p, nbA, c->rzs.p + rz_slot,
+                         c->p.p, c->q.p, c->x.p, c->r.p, c->scal.p, c->alphas.p + it);
+      HIP_LAUNCH_CHECK();
+      HIP_CHECK(hipMemcpyAsync(c->hscal, c->scal.p, (kScalarHeader + grid_m) * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      if (c->hscal[kStop] != 0.0) {
+        info = (int)c->hscal[kInfo];
+        break;
+      }
+      ++it;
+      const double rr = host_sum(c->hscal + kScalarHeader, grid_m);
+      const double rec = std::sqrt(rr) / scale;
+      const bool replace = replacement > 0 && it % replacement == 0;
+      if (replace || rec <= tol) {
+        nb = device_residual(c, c->rhs.p, c->x.p, c->r.p, c->rt.p, c->partD.p);
+        read_partials(c, c->partD.p, nb, 2, tmp);
+        ++checks;
+        const double tn = std::sqrt(tmp[0]);
+        const double rel = tn / scale;
+        hist.push_back(rel);
+        if (!std::isfinite(rel)) { info = -4; break; }
+        if (rel <= tol) {
+          copy_dev(c, c->r.p, c->rt.p, m);
+          info = 0;
+          break;
+        }
+        const double gap = std::sqrt(tmp[1]);
+        if (gap > gap_tol * std::max(tn, tol * scale)) {
+          copy_dev(c, c->r.p, c->rt.p, m);
+          ++replacements;
+          factor_apply(c, c->r.p, c->z.p);
+          nb = device_dot(c, c->r.p, c->z.p, m, c->partD.p);
+          read_partials(c, c->partD.p, nb, 1, tmp);
+          rz = tmp[0];
+          if (!std::isfinite(rz) || rz <= 0.0) { info = -5; break; }
+          HIP_CHECK(hipMemcpyAsync(c->rzs.p + it, &rz, sizeof(double), hipMemcpyHostToDevice, s));
+          rz_slot = it;
+          copy_dev(c, c->p.p, c->z.p, m);
+          alpha_start = it;
+          beta_start = (int)c->hscal[kNBeta];
+          continue;
+        }
+      }
+      factor_apply(c, c->r.p, c->z.p);
+      const int nbC = device_dot(c, c->r.p, c->z.p, m, c->partC.p);
+      hipLaunchKernelGGL(k_pcg_direction, dim3(grid_m), dim3(kBlock), 0, s, m, c->partC.p, nbC,
+                         c->rzs.p + rz_slot, c->rzs.p + it, c->z.p, c->p.p, c->scal.p, c->betas.p);
+      HIP_LAUNCH_CHECK();
+      rz_slot = it;
+    }
+    // pick up a direction-kernel failure raised after the last synchronisation
+    HIP_CHECK(hipMemcpyAsync(c->hscal, c->scal.p, kScalarHeader * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (info > 0 && c->hscal[kStop] != 0.0) info = (int)c->hscal[kInfo];
+    const int nbeta_total = (int)c->hscal[kNBeta];
+    // final explicit residual (SSP:558-561), kept in rt
+    nb = device_residual(c, c->rhs.p, c->x.p, nullptr, c->rt.p, c->partD.p);
+    read_partials(c, c->partD.p, nb, 1, tmp);
+    const double relative = std::sqrt(tmp[0]) / scale;
+    if (info > 0 && relative <= tol) info = 0;
+    HIP_CHECK(hipEventRecord(c->ev1, s));
+    const int na = std::max(0, it - alpha_start);
+    const int nbt = std::max(0, nbeta_total - beta_start);
+    if (alphas_out && na) HIP_CHECK(hipMemcpyAsync(alphas_out, c->alphas.p + alpha_start, na * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (betas_out && nbt) HIP_CHECK(hipMemcpyAsync(betas_out, c->betas.p + beta_start, nbt * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_pcg_ms = ms;
+    if (history_out) std::copy(hist.begin(), hist.end(), history_out);
+    res->info = info;
+    res->iterations = it;
+    res->relative_residual = relative;
+    res->replacements = replacements;
+    res->residual_checks = checks;
+    res->n_alpha = na;
+    res->n_beta = nbt;
+    res->n_history = (int)hist.size();
+    res->stagnated = 0;
+    c->have_solution = true;
+    return PSG_OK;
+  });
+}
+
+int psg_solve_finish(psg_ctx* c, double* dx, double* dy) {
+  return guarded([&]() -> int {
+    if (!c || !dx || !dy) throw Error(PSG_ERR_ARGUMENT, "psg_solve_finish: null argument");
+    if (!c->have_solution) throw Error(PSG_ERR_STATE, "psg_solve_finish: no PCG solution");
+    // dx = W (r1 + A^T dy)  (SSP:716)
+    launch_gather<double>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval.p, c->x.p, c->w.p, c->r1_n.p,
+                          c->dx_n.p, c->stream);
+    c->dx_n.download(dx, c->n, c->stream);
+    c->x.download(dy, c->m, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PSG_OK;
+  });
+}
+
+int psg_pcg_residual(psg_ctx* c, double* residual) {
+  return guarded([&]() -> int {
+    if (!c || !residual) throw Error(PSG_ERR_ARGUMENT, "psg_pcg_residual: null argument");
+    if (!c->have_solution) throw Error(PSG_ERR_STATE, "psg_pcg_residual: no PCG run");
+    c->rt.download(residual, c->m, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PSG_OK;
+  });
+}
+
+int psg_last_timings(psg_ctx* c, double* pcg_ms, double* factor_ms) {
+  return guarded([&]() -> int {
+    if (!c) throw Error(PSG_ERR_ARGUMENT, "psg_last_timings: null context");
+    if (pcg_ms) *pcg_ms = c->last_pcg_ms;
+    if (factor_ms) *factor_ms = c->last_factor_ms;
+    return PSG_OK;
+  });
+}
+
+}  // extern "C"

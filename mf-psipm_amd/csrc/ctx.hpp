@@ -1,0 +1,66 @@
+// Device state of one LP (the psg_ctx behind the C-ABI).
+#pragma once
+#include "device.hpp"
+#include "symbolic.hpp"
+
+namespace psg {
+
+struct FactorState;  // factor.hip
+
+// layout of the small host-visible scalar block (doubles)
+enum ScalarSlot {
+  kStop = 0,     // nonzero: PCG vector kernels skip their work
+  kInfo = 1,     // PCG info code recorded on the device
+  kAlpha = 2,
+  kCurv = 3,
+  kNBeta = 4,    // number of accepted beta values
+  kBeta = 5,
+  kScalarHeader = 16,  // partials of the last axpy follow the header
+};
+
+}  // namespace psg
+
+struct psg_ctx {
+  int m = 0, n = 0;
+  int64_t nnz = 0;
+  int precision = PSG_FP64;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // A as CSC (column gather: A^T v) and CSR (row gather: A t)
+  psg::DevBuf<int> cptr, crow, rptr, rcol;
+  psg::DevBuf<double> cval, rval;
+  psg::DevBuf<float> cval32, rval32;
+  int g_csc = 8, g_csr = 32;  // lanes per column / per row
+  // Newton matrix
+  psg::DevBuf<double> theta, w;
+  double delta = 0.0;
+  bool have_weights = false;
+  // n-vectors
+  psg::DevBuf<double> t_n, r1_n, dx_n;
+  // m-vectors
+  psg::DevBuf<double> rhs, x, r, z, p, q, rt, vin, vout;
+  psg::DevBuf<double> d_full, d_core, dsc;
+  psg::DevBuf<uint8_t> mask;
+  psg::DevBuf<int> cand;
+  psg::DevBuf<double> cand_out;
+  // reductions and PCG scalars
+  psg::DevBuf<double> partA, partC, partD;  // per-block partials
+  psg::DevBuf<double> scal;                 // header + partials of the axpy kernel
+  psg::DevBuf<double> rzs, alphas, betas;
+  double* hscal = nullptr;                  // pinned mirror of scal
+  bool have_rhs = false, have_solution = false;
+  double last_pcg_ms = 0.0, last_factor_ms = 0.0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  psg::FactorState* fs = nullptr;
+};
+
+namespace psg {
+// factor.hip
+void factor_destroy(psg_ctx* ctx);
+int factor_numeric(psg_ctx* ctx, const Plan& plan, double eta, double* nnz_factor);
+bool factor_ready(const psg_ctx* ctx);
+// z = D P^T L^-T L^-1 P D r on device vectors (m), on ctx->stream
+void factor_apply(psg_ctx* ctx, const double* r, double* z);
+// context.hip helpers used by factor.hip
+void diag_terms_device(psg_ctx* ctx, const uint8_t* mask_dev, double* full, double* core);
+}  // namespace psg

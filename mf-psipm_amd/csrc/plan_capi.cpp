@@ -1,0 +1,104 @@
+// C-ABI of the host symbolic analysis and the library-wide error state.
+#include <algorithm>
+#include <cstring>
+
+#include "errors.hpp"
+#include "symbolic.hpp"
+
+
+namespace psg {
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+template <class T, class U>
+static void put(T* dst, const std::vector<U>& src) {
+  if (dst) for (size_t i = 0; i < src.size(); ++i) dst[i] = (T)src[i];
+}
+}  // namespace psg
+using psg::put;
+
+extern "C" {
+
+const char* psg_last_error(void) { return psg::g_last_error.c_str(); }
+
+int psg_version(void) { return 10000; }
+
+int psg_plan_create(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowind,
+                    const int32_t* core, int64_t ncore, int32_t ordering, int32_t max_width,
+                    double dense_factor, psg_plan** out) {
+  return psg::guarded([&]() -> int {
+    if (!out || m < 0 || n < 0 || ncore < 0 || (ncore && (!core || !colptr || !rowind)))
+      throw psg::Error(PSG_ERR_ARGUMENT, "psg_plan_create: invalid arguments");
+    if (m >= (int64_t)1 << 31 || n >= (int64_t)1 << 31)
+      throw psg::Error(PSG_ERR_ARGUMENT, "psg_plan_create: dimensions exceed int32");
+    for (int64_t j = 0; j < ncore; ++j)
+      if (core[j] < 0 || core[j] >= n) throw psg::Error(PSG_ERR_ARGUMENT, "psg_plan_create: core index out of range");
+    for (int64_t j = 0; j < ncore; ++j)
+      for (int64_t p = colptr[core[j]]; p < colptr[core[j] + 1]; ++p)
+        if (rowind[p] < 0 || rowind[p] >= m) throw psg::Error(PSG_ERR_ARGUMENT, "psg_plan_create: row index out of range");
+    psg::PlanOptions opt;
+    opt.ordering = ordering;
+    opt.max_width = std::max(1, (int)max_width);
+    opt.dense_factor = dense_factor;
+    auto* p = new psg_plan;
+    try {
+      p->plan = psg::build_plan((int)m, colptr, rowind, core, (int)ncore, opt);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+    return PSG_OK;
+  });
+}
+
+int psg_plan_info(const psg_plan* plan, double* info) {
+  return psg::guarded([&]() -> int {
+    if (!plan || !info) throw psg::Error(PSG_ERR_ARGUMENT, "psg_plan_info: null argument");
+    const auto& P = plan->plan;
+    info[0] = P.m;
+    info[1] = P.nsuper;
+    info[2] = P.nlevels;
+    info[3] = (double)P.nnz_panels;
+    info[4] = (double)P.nnz_L;
+    info[5] = (double)P.pair_col.size();
+    info[6] = (double)P.seg_target.size();
+    info[7] = P.max_width;
+    info[8] = (double)P.max_rows;
+    info[9] = P.factor_flops;
+    info[10] = (double)P.rows.size();
+    info[11] = 0.0;
+    return PSG_OK;
+  });
+}
+
+int psg_plan_export(const psg_plan* plan, int32_t* perm, int32_t* sfirst, int64_t* rptr,
+                    int32_t* rows, int64_t* poff, int32_t* sparent, int32_t* level_ptr,
+                    int32_t* level_nodes, int64_t* diag_index, int64_t* seg_target,
+                    int64_t* seg_ptr, int32_t* pair_col, int64_t* pair_a, int64_t* pair_b) {
+  return psg::guarded([&]() -> int {
+    if (!plan) throw psg::Error(PSG_ERR_ARGUMENT, "psg_plan_export: null plan");
+    const auto& P = plan->plan;
+    put(perm, P.perm);
+    put(sfirst, P.sfirst);
+    put(rptr, P.rptr);
+    put(rows, P.rows);
+    put(poff, P.poff);
+    put(sparent, P.sparent);
+    put(level_ptr, P.level_ptr);
+    put(level_nodes, P.level_nodes);
+    put(diag_index, P.diag_index);
+    put(seg_target, P.seg_target);
+    put(seg_ptr, P.seg_ptr);
+    put(pair_col, P.pair_col);
+    put(pair_a, P.pair_a);
+    put(pair_b, P.pair_b);
+    return PSG_OK;
+  });
+}
+
+int psg_plan_destroy(psg_plan* plan) {
+  delete plan;
+  return PSG_OK;
+}
+
+}  // extern "C"

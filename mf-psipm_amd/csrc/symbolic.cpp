@@ -1,0 +1,579 @@
+// Host symbolic analysis of the active-column preconditioner matrix
+//     M = A_C diag(w_C) A_C^T + diag(d)            (m x m, SPD)
+// whose sparsity pattern is the union of one clique per active column.
+//
+// Replaces CHOLMOD's analyze step (psipm_active_pool.py:668-671, order
+// "best"; pattern-reuse test psipm_active_pool.py:842-846).  CHOLMOD and
+// METIS are not available, so every step here is this project's own:
+//
+//   1. fill-reducing ordering: minimum degree on the quotient graph whose
+//      initial elements ARE the active columns (M is never formed), with
+//      approximate external degrees, element absorption, supervariable
+//      detection (mass elimination) and dense-row postponement;
+//   2. elimination tree of M from the column lists (M = B B^T, B = A_C);
+//   3. postorder relabelling;
+//   4. supernode partition (fundamental supernodes, width capped so dense
+//      panels stay LDS-sized) and row structures computed per supernode,
+//      using Struct(L_k) = U{rows(j) : first(j) = k} u U{Struct(child)};
+//   5. level sets of the supernodal tree for level-scheduled kernels;
+//   6. the scatter map that assembles w_j a_ij a_kj straight into the
+//      column-major supernodal panels (sorted by target, so the device sum
+//      is deterministic and atomic-free).
+//
+// Everything is plain C++ (no HIP), so it builds and is tested on the CPU.
+#include "symbolic.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+
+namespace psg {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// 1. Minimum-degree ordering on the column-element quotient graph.
+// ---------------------------------------------------------------------------
+struct MinDegree {
+  int nv;                                   // number of variables (rows of M)
+  std::vector<std::vector<int>> elems_of;   // E_i : elements adjacent to variable i
+  std::vector<std::vector<int>> vars_of;    // L_e : variables of element e
+  std::vector<char> elem_alive;
+  std::vector<int> weight;                  // supervariable size (0 = absorbed/eliminated)
+  std::vector<int> degree;                  // approximate external degree
+  std::vector<int> head, next, prev;        // degree buckets (doubly linked)
+  std::vector<int> member_next;             // linked list of variables merged into a principal one
+  std::vector<int> member_tail;
+  std::vector<char> eliminated;
+  std::vector<int> mark;                    // scratch marks
+  std::vector<int> wext;                    // |L_e \ Lp| scratch
+  int stamp = 0;
+  int min_deg = 0;
+
+  void bucket_insert(int i) {
+    int d = degree[i];
+    next[i] = head[d];
+    prev[i] = -1;
+    if (head[d] >= 0) prev[head[d]] = i;
+    head[d] = i;
+    if (d < min_deg) min_deg = d;
+  }
+  void bucket_remove(int i) {
+    int d = degree[i];
+    if (prev[i] >= 0) next[prev[i]] = next[i]; else head[d] = next[i];
+    if (next[i] >= 0) prev[next[i]] = prev[i];
+    next[i] = prev[i] = -1;
+  }
+  int new_stamp() {
+    if (++stamp == 0x7fffffff) {
+      std::fill(mark.begin(), mark.end(), 0);
+      stamp = 1;
+    }
+    return stamp;
+  }
+};
+
+}  // namespace
+
+std::vector<int> minimum_degree_order(int m, int ncols, const int64_t* colptr,
+                                      const int* rowind, double dense_factor) {
+  // Variables 0..m-1; elements 0..ncols-1 are the active columns, element
+  // ncols + p is the one created when variable p is eliminated.
+  MinDegree g;
+  g.nv = m;
+  g.elems_of.assign(m, {});
+  g.vars_of.assign((size_t)ncols + m, {});
+  g.elem_alive.assign((size_t)ncols + m, 0);
+  for (int j = 0; j < ncols; ++j) {
+    int64_t b = colptr[j], e = colptr[j + 1];
+    if (e - b < 2) continue;  // singleton columns add no off-diagonal entry
+    auto& L = g.vars_of[j];
+    L.assign(rowind + b, rowind + e);
+    std::sort(L.begin(), L.end());
+    L.erase(std::unique(L.begin(), L.end()), L.end());
+    if (L.size() < 2) { L.clear(); continue; }
+    g.elem_alive[j] = 1;
+    for (int i : L) g.elems_of[i].push_back(j);
+  }
+  g.weight.assign(m, 1);
+  g.degree.assign(m, 0);
+  g.head.assign(m + 1, -1);
+  g.next.assign(m, -1);
+  g.prev.assign(m, -1);
+  g.member_next.assign(m, -1);
+  g.member_tail.resize(m);
+  std::iota(g.member_tail.begin(), g.member_tail.end(), 0);
+  g.eliminated.assign(m, 0);
+  g.mark.assign(std::max<size_t>((size_t)m, (size_t)ncols + m), 0);
+  g.wext.assign((size_t)ncols + m, -1);
+
+  // initial (over-)estimate of the external degree: sum of clique sizes,
+  // capped by m - 1; exact degrees arrive as soon as a neighbour is pivoted.
+  std::vector<int> order;
+  order.reserve(m);
+  std::vector<int> dense;
+  int dense_limit = std::max(16, (int)(dense_factor * std::sqrt((double)m)));
+  int remaining = m;
+  for (int i = 0; i < m; ++i) {
+    long long d = 0;
+    int s = g.new_stamp();
+    for (int e : g.elems_of[i])
+      for (int v : g.vars_of[e])
+        if (v != i && g.mark[v] != s) { g.mark[v] = s; ++d; }
+    g.degree[i] = (int)std::min<long long>(d, m - 1);
+  }
+  g.min_deg = m;
+  for (int i = 0; i < m; ++i) {
+    if (g.degree[i] > dense_limit && dense_factor > 0) {
+      dense.push_back(i);
+      g.weight[i] = 0;  // postponed: treated as removed from the graph
+      --remaining;
+      continue;
+    }
+    g.bucket_insert(i);
+  }
+  // strip postponed variables from element lists
+  if (!dense.empty()) {
+    for (size_t e = 0; e < (size_t)ncols; ++e) {
+      if (!g.elem_alive[e]) continue;
+      auto& L = g.vars_of[e];
+      L.erase(std::remove_if(L.begin(), L.end(), [&](int v) { return g.weight[v] == 0; }), L.end());
+    }
+  }
+
+  std::vector<int> Lp;
+  std::vector<int> touched_elems;
+  std::vector<std::pair<long long, int>> hashed;
+  while (remaining > 0) {
+    while (g.min_deg <= m && g.head[g.min_deg] < 0) ++g.min_deg;
+    int p = g.head[g.min_deg];
+    g.bucket_remove(p);
+    // ---- form the new element Lp = U_{e in E_p} L_e \ {p} -------------------
+    int s = g.new_stamp();
+    g.mark[p] = s;
+    Lp.clear();
+    int lp_weight = 0;
+    for (int e : g.elems_of[p]) {
+      if (!g.elem_alive[e]) continue;
+      for (int v : g.vars_of[e]) {
+        if (g.weight[v] > 0 && !g.eliminated[v] && g.mark[v] != s) {
+          g.mark[v] = s;
+          Lp.push_back(v);
+          lp_weight += g.weight[v];
+        }
+      }
+      g.elem_alive[e] = 0;  // absorbed into the new element
+      std::vector<int>().swap(g.vars_of[e]);
+    }
+    // eliminate p (and the variables merged into it)
+    g.eliminated[p] = 1;
+    for (int v = p; v >= 0; v = g.member_next[v]) order.push_back(v);
+    remaining -= g.weight[p];
+    const int pe = ncols + p;
+    g.vars_of[pe] = Lp;
+    g.elem_alive[pe] = 1;
+    std::vector<int>().swap(g.elems_of[p]);
+
+    // ---- update element lists of i in Lp, compute |L_e \ Lp| ---------------
+    touched_elems.clear();
+    for (int i : Lp) {
+      auto& E = g.elems_of[i];
+      size_t k = 0;
+      for (int e : E)
+        if (g.elem_alive[e]) E[k++] = e;
+      E.resize(k);
+      for (int e : E) {
+        if (g.wext[e] < 0) {
+          int wsum = 0;
+          for (int v : g.vars_of[e]) if (g.weight[v] > 0 && !g.eliminated[v]) wsum += g.weight[v];
+          g.wext[e] = wsum;
+          touched_elems.push_back(e);
+        }
+        g.wext[e] -= g.weight[i];
+      }
+    }
+    // aggressive absorption: elements fully inside Lp disappear
+    for (int e : touched_elems) {
+      if (g.wext[e] <= 0) {
+        g.elem_alive[e] = 0;
+        std::vector<int>().swap(g.vars_of[e]);
+      }
+    }
+    // ---- approximate degrees and supervariable hashing ---------------------
+    hashed.clear();
+    for (int i : Lp) {
+      auto& E = g.elems_of[i];
+      size_t k = 0;
+      long long ext = 0;
+      long long h = 0;
+      for (int e : E) {
+        if (!g.elem_alive[e]) continue;
+        E[k++] = e;
+        ext += g.wext[e];
+        h += e;
+      }
+      E.resize(k);
+      E.push_back(pe);
+      h += pe;
+      g.bucket_remove(i);
+      long long d1 = (long long)g.degree[i] + lp_weight - g.weight[i];
+      long long d2 = ext + lp_weight - g.weight[i];
+      long long d3 = (long long)remaining - g.weight[i];
+      long long d = std::min(d1, std::min(d2, d3));
+      if (d < 0) d = 0;
+      g.degree[i] = (int)d;
+      hashed.emplace_back(h * 1000003LL + (long long)E.size(), i);
+    }
+    for (int e : touched_elems) g.wext[e] = -1;
+    // ---- supervariable detection: identical element lists ------------------
+    std::sort(hashed.begin(), hashed.end());
+    for (size_t a = 0; a < hashed.size();) {
+      size_t b = a + 1;
+      while (b < hashed.size() && hashed[b].first == hashed[a].first) ++b;
+      for (size_t x = a; x < b; ++x) {
+        int i = hashed[x].second;
+        if (g.weight[i] == 0) continue;
+        auto& Ei = g.elems_of[i];
+        std::sort(Ei.begin(), Ei.end());
+        for (size_t y = x + 1; y < b; ++y) {
+          int j = hashed[y].second;
+          if (g.weight[j] == 0) continue;
+          auto& Ej = g.elems_of[j];
+          if (Ej.size() != Ei.size()) continue;
+          std::sort(Ej.begin(), Ej.end());
+          if (Ej != Ei) continue;
+          // merge j into i: j stops being external to i
+          g.degree[i] = std::max(0, g.degree[i] - g.weight[j]);
+          g.weight[i] += g.weight[j];
+          g.weight[j] = 0;
+          g.member_next[g.member_tail[i]] = j;
+          g.member_tail[i] = g.member_tail[j];
+          std::vector<int>().swap(Ej);
+        }
+      }
+      a = b;
+    }
+    // external degree excludes the supervariable itself
+    for (auto& hv : hashed) {
+      int i = hv.second;
+      if (g.weight[i] == 0) continue;
+      int d = std::min(g.degree[i], remaining - g.weight[i]);
+      g.degree[i] = std::max(0, d);
+      g.bucket_insert(i);
+    }
+    // prune absorbed variables from the new element
+    auto& Lnew = g.vars_of[pe];
+    size_t k = 0;
+    for (int v : Lnew) if (g.weight[v] > 0) Lnew[k++] = v;
+    Lnew.resize(k);
+    if (Lnew.empty()) g.elem_alive[pe] = 0;
+  }
+  for (int v : dense) order.push_back(v);
+  if ((int)order.size() != m) throw std::runtime_error("minimum degree produced an incomplete order");
+  return order;
+}
+
+// ---------------------------------------------------------------------------
+// 2-6. Elimination tree, postorder, supernodes, levels, scatter map.
+// ---------------------------------------------------------------------------
+namespace {
+
+// etree of B B^T with B given by column lists in permuted row numbering.
+std::vector<int> etree_of_gram(int m, const std::vector<std::vector<int>>& cols_of_node) {
+  std::vector<int> parent(m, -1), ancestor(m, -1);
+  std::vector<int> prev_in_col;  // sized on demand
+  int ncols = 0;
+  for (auto& v : cols_of_node) for (int j : v) ncols = std::max(ncols, j + 1);
+  prev_in_col.assign(ncols, -1);
+  for (int k = 0; k < m; ++k) {
+    for (int j : cols_of_node[k]) {
+      int i = prev_in_col[j];
+      while (i != -1 && i < k) {
+        int inext = ancestor[i];
+        ancestor[i] = k;
+        if (inext == -1) { parent[i] = k; break; }
+        i = inext;
+      }
+      prev_in_col[j] = k;
+    }
+  }
+  return parent;
+}
+
+std::vector<int> postorder(int m, const std::vector<int>& parent) {
+  std::vector<int> head(m, -1), next(m, -1), post;
+  post.reserve(m);
+  for (int j = m - 1; j >= 0; --j) {
+    if (parent[j] == -1) continue;
+    next[j] = head[parent[j]];
+    head[parent[j]] = j;
+  }
+  std::vector<int> stack;
+  for (int r = 0; r < m; ++r) {
+    if (parent[r] != -1) continue;
+    stack.push_back(r);
+    while (!stack.empty()) {
+      int p = stack.back();
+      int c = head[p];
+      if (c == -1) {
+        stack.pop_back();
+        post.push_back(p);
+      } else {
+        head[p] = next[c];
+        stack.push_back(c);
+      }
+    }
+  }
+  return post;
+}
+
+}  // namespace
+
+Plan build_plan(int m, const int64_t* a_colptr, const int* a_rowind, const int* core,
+                int ncore, const PlanOptions& opt) {
+  static std::atomic<uint64_t> next_serial{1};
+  Plan P;
+  P.serial = next_serial++;
+  P.m = m;
+  P.core.assign(core, core + ncore);
+  // local CSC of the active columns; nz positions are kept in A's numbering
+  std::vector<int64_t> colptr(ncore + 1, 0);
+  for (int j = 0; j < ncore; ++j) colptr[j + 1] = colptr[j] + (a_colptr[core[j] + 1] - a_colptr[core[j]]);
+  std::vector<int> rowind(colptr[ncore]);
+  std::vector<int64_t> nzpos(colptr[ncore]);
+  for (int j = 0; j < ncore; ++j)
+    for (int64_t p = a_colptr[core[j]], q = colptr[j]; p < a_colptr[core[j] + 1]; ++p, ++q) {
+      rowind[q] = a_rowind[p];
+      nzpos[q] = p;
+    }
+  // ---- 1. ordering ----------------------------------------------------------
+  std::vector<int> order;
+  if (opt.ordering == 0) {
+    order.resize(m);
+    std::iota(order.begin(), order.end(), 0);
+  } else {
+    order = minimum_degree_order(m, ncore, colptr.data(), rowind.data(), opt.dense_factor);
+  }
+  std::vector<int> inv(m);
+  for (int k = 0; k < m; ++k) inv[order[k]] = k;
+
+  // column lists in the permuted numbering, and their transpose
+  std::vector<std::vector<int>> col_rows(ncore);
+  std::vector<std::vector<int>> cols_of_node(m);
+  for (int j = 0; j < ncore; ++j) {
+    auto& r = col_rows[j];
+    for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) r.push_back(inv[rowind[p]]);
+    std::sort(r.begin(), r.end());
+    r.erase(std::unique(r.begin(), r.end()), r.end());
+    if (r.size() < 2) continue;
+    for (int i : r) cols_of_node[i].push_back(j);
+  }
+  // ---- 2-3. etree + postorder ----------------------------------------------
+  std::vector<int> parent = etree_of_gram(m, cols_of_node);
+  std::vector<int> post = postorder(m, parent);
+  std::vector<int> relabel(m);
+  for (int k = 0; k < m; ++k) relabel[post[k]] = k;
+  P.perm.resize(m);
+  for (int k = 0; k < m; ++k) P.perm[k] = order[post[k]];
+  P.iperm.resize(m);
+  for (int k = 0; k < m; ++k) P.iperm[P.perm[k]] = k;
+  std::vector<int> par(m, -1);
+  for (int k = 0; k < m; ++k) {
+    int old = post[k];
+    par[k] = parent[old] < 0 ? -1 : relabel[parent[old]];
+  }
+  for (auto& r : col_rows) {
+    for (auto& i : r) i = relabel[i];
+    std::sort(r.begin(), r.end());
+  }
+  // own(k): columns whose first (smallest) row is k
+  std::vector<int> own_ptr(m + 1, 0);
+  for (int j = 0; j < ncore; ++j) if (col_rows[j].size() >= 2) own_ptr[col_rows[j][0] + 1]++;
+  for (int k = 0; k < m; ++k) own_ptr[k + 1] += own_ptr[k];
+  std::vector<int> own(own_ptr[m]);
+  {
+    std::vector<int> fill(own_ptr.begin(), own_ptr.end() - 1);
+    for (int j = 0; j < ncore; ++j) if (col_rows[j].size() >= 2) own[fill[col_rows[j][0]]++] = j;
+  }
+  std::vector<int> nchild(m, 0);
+  for (int k = 0; k < m; ++k) if (par[k] >= 0) nchild[par[k]]++;
+
+  // ---- 4. supernodes --------------------------------------------------------
+  const int wmax = std::max(1, opt.max_width);
+  std::vector<int> sfirst;
+  std::vector<std::vector<int>> srows;  // full row structure (starts with own cols)
+  std::vector<int> mark(m, -1);
+  std::vector<int> snode_of(m, -1);
+  // children supernodes by parent column
+  std::vector<std::vector<int>> child_super(m);
+  int cur = -1;
+  for (int k = 0; k < m; ++k) {
+    // does k extend the current supernode?
+    bool extend = false;
+    if (cur >= 0 && k > 0 && par[k - 1] == k && nchild[k] == 1 &&
+        snode_of[k - 1] == cur && (k - sfirst[cur]) < wmax) {
+      // own(k) must be inside Struct(cur)
+      const auto& R = srows[cur];
+      extend = true;
+      for (int q = own_ptr[k]; q < own_ptr[k + 1] && extend; ++q)
+        for (int i : col_rows[own[q]])
+          if (!std::binary_search(R.begin(), R.end(), i)) { extend = false; break; }
+    }
+    if (extend) {
+      snode_of[k] = cur;
+      continue;
+    }
+    // close `cur`: register as child of its parent column
+    if (cur >= 0) {
+      int last = k - 1;
+      if (par[last] >= 0) child_super[par[last]].push_back(cur);
+    }
+    // new supernode at k
+    cur = (int)sfirst.size();
+    sfirst.push_back(k);
+    snode_of[k] = cur;
+    std::vector<int> R;
+    R.push_back(k);
+    mark[k] = cur;
+    for (int q = own_ptr[k]; q < own_ptr[k + 1]; ++q)
+      for (int i : col_rows[own[q]])
+        if (i > k && mark[i] != cur) { mark[i] = cur; R.push_back(i); }
+    for (int c : child_super[k]) {
+      for (int i : srows[c])
+        if (i > k && mark[i] != cur) { mark[i] = cur; R.push_back(i); }
+    }
+    std::vector<int>().swap(child_super[k]);
+    std::sort(R.begin(), R.end());
+    srows.push_back(std::move(R));
+  }
+  if (cur >= 0 && par[m - 1] >= 0) child_super[par[m - 1]].push_back(cur);
+  const int ns = (int)sfirst.size();
+  sfirst.push_back(m);
+  // the first w rows of each structure must be the supernode's own columns;
+  // extension guaranteed rows k..last are present (they are parents).
+  P.nsuper = ns;
+  P.sfirst = sfirst;
+  P.rptr.assign(ns + 1, 0);
+  P.poff.assign(ns + 1, 0);
+  for (int s = 0; s < ns; ++s) {
+    int w = sfirst[s + 1] - sfirst[s];
+    auto& R = srows[s];
+    for (int c = 0; c < w; ++c)
+      if ((int)R.size() <= c || R[c] != sfirst[s] + c)
+        throw std::runtime_error("supernode structure does not start with its columns");
+    P.rptr[s + 1] = P.rptr[s] + (int64_t)R.size();
+    P.poff[s + 1] = P.poff[s] + (int64_t)R.size() * w;
+  }
+  P.rows.resize(P.rptr[ns]);
+  for (int s = 0; s < ns; ++s) std::copy(srows[s].begin(), srows[s].end(), P.rows.begin() + P.rptr[s]);
+  P.snode_of = snode_of;
+  P.sparent.assign(ns, -1);
+  for (int s = 0; s < ns; ++s) {
+    int w = sfirst[s + 1] - sfirst[s];
+    int64_t nr = P.rptr[s + 1] - P.rptr[s];
+    if (nr > w) P.sparent[s] = snode_of[P.rows[P.rptr[s] + w]];
+  }
+  // ---- 5. levels (leaves = level 0) ------------------------------------------
+  std::vector<int> level(ns, 0);
+  int nlev = 0;
+  for (int s = 0; s < ns; ++s) {  // children precede parents (postorder)
+    int p = P.sparent[s];
+    if (p >= 0) level[p] = std::max(level[p], level[s] + 1);
+    nlev = std::max(nlev, level[s] + 1);
+  }
+  P.level_ptr.assign(nlev + 1, 0);
+  for (int s = 0; s < ns; ++s) P.level_ptr[level[s] + 1]++;
+  for (int l = 0; l < nlev; ++l) P.level_ptr[l + 1] += P.level_ptr[l];
+  P.level_nodes.resize(ns);
+  {
+    std::vector<int> fill(P.level_ptr.begin(), P.level_ptr.end() - 1);
+    for (int s = 0; s < ns; ++s) P.level_nodes[fill[level[s]]++] = s;
+  }
+  P.nlevels = nlev;
+  // ---- 6. scatter map --------------------------------------------------------
+  // diagonal slot of every permuted column
+  P.diag_index.resize(m);
+  for (int s = 0; s < ns; ++s) {
+    int w = sfirst[s + 1] - sfirst[s];
+    int64_t nr = P.rptr[s + 1] - P.rptr[s];
+    for (int c = 0; c < w; ++c) P.diag_index[sfirst[s] + c] = P.poff[s] + (int64_t)c * nr + c;
+  }
+  struct Pair { int64_t target; int col; int64_t a, b; };
+  std::vector<Pair> pairs;
+  size_t npairs = 0;
+  for (int j = 0; j < ncore; ++j) {
+    size_t q = (size_t)(colptr[j + 1] - colptr[j]);
+    npairs += q * (q - 1) / 2;
+  }
+  pairs.reserve(npairs);
+  std::vector<std::pair<int, int64_t>> loc;  // (permuted row, nz position)
+  for (int j = 0; j < ncore; ++j) {
+    loc.clear();
+    for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) loc.emplace_back(P.iperm[rowind[p]], nzpos[p]);
+    std::sort(loc.begin(), loc.end());
+    for (size_t b = 0; b < loc.size(); ++b) {
+      int col = loc[b].first;
+      int s = snode_of[col];
+      int c = col - sfirst[s];
+      int64_t nr = P.rptr[s + 1] - P.rptr[s];
+      const int* R = P.rows.data() + P.rptr[s];
+      for (size_t a = b + 1; a < loc.size(); ++a) {
+        int row = loc[a].first;
+        if (row == col) continue;  // duplicate entries are summed upstream
+        const int* it = std::lower_bound(R, R + nr, row);
+        if (it == R + nr || *it != row) throw std::runtime_error("scatter target missing from structure");
+        int64_t t = P.poff[s] + (int64_t)c * nr + (it - R);
+        pairs.push_back({t, core[j], loc[a].second, loc[b].second});
+      }
+    }
+  }
+  std::sort(pairs.begin(), pairs.end(), [](const Pair& x, const Pair& y) {
+    return x.target < y.target || (x.target == y.target && (x.col < y.col || (x.col == y.col && x.a < y.a)));
+  });
+  P.seg_ptr.clear();
+  P.seg_target.clear();
+  P.pair_col.resize(pairs.size());
+  P.pair_a.resize(pairs.size());
+  P.pair_b.resize(pairs.size());
+  for (size_t k = 0; k < pairs.size(); ++k) {
+    if (k == 0 || pairs[k].target != pairs[k - 1].target) {
+      P.seg_ptr.push_back((int64_t)k);
+      P.seg_target.push_back(pairs[k].target);
+    }
+    P.pair_col[k] = pairs[k].col;
+    P.pair_a[k] = pairs[k].a;
+    P.pair_b[k] = pairs[k].b;
+  }
+  P.seg_ptr.push_back((int64_t)pairs.size());
+  P.nnz_panels = P.poff[ns];
+  // nnz(L) proper: lower-trapezoid entries of every panel
+  int64_t nnzl = 0;
+  int maxw = 0;
+  int64_t maxr = 0;
+  for (int s = 0; s < ns; ++s) {
+    int w = sfirst[s + 1] - sfirst[s];
+    int64_t nr = P.rptr[s + 1] - P.rptr[s];
+    nnzl += nr * w - (int64_t)w * (w - 1) / 2;
+    maxw = std::max(maxw, w);
+    maxr = std::max(maxr, nr);
+  }
+  P.nnz_L = nnzl;
+  P.max_width = maxw;
+  P.max_rows = maxr;
+  // flops of the numeric factor: sum over columns of c_j^2
+  double flops = 0.0;
+  for (int s = 0; s < ns; ++s) {
+    int w = sfirst[s + 1] - sfirst[s];
+    int64_t nr = P.rptr[s + 1] - P.rptr[s];
+    for (int c = 0; c < w; ++c) {
+      double cc = (double)(nr - c);
+      flops += cc * cc;
+    }
+  }
+  P.factor_flops = flops;
+  return P;
+}
+
+}  // namespace psg

@@ -1,0 +1,55 @@
+// Host symbolic analysis of the active-column preconditioner (plain C++).
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+namespace psg {
+
+struct PlanOptions {
+  int ordering = 1;          // 0 natural, 1 minimum degree
+  int max_width = 64;        // supernode column cap (dense panel kernels use LDS)
+  double dense_factor = 10;  // rows with degree > dense_factor*sqrt(m) are ordered last
+};
+
+// Supernodal plan of M = A_C W_C A_C^T + diag, in the permuted numbering
+// P M P^T (perm[k] = original row of pivot k).
+struct Plan {
+  uint64_t serial = 0;     // unique per built plan (device cache key)
+  int m = 0;
+  std::vector<int> core;            // active columns (indices into A)
+  std::vector<int> perm, iperm;
+  int nsuper = 0;
+  std::vector<int> sfirst;          // nsuper+1: first column of each supernode
+  std::vector<int64_t> rptr;        // nsuper+1: row-structure offsets
+  std::vector<int> rows;            // row structures (sorted, own columns first)
+  std::vector<int64_t> poff;        // nsuper+1: panel offsets (column-major nr x w)
+  std::vector<int> sparent;         // supernodal elimination tree
+  std::vector<int> snode_of;        // column -> supernode
+  int nlevels = 0;
+  std::vector<int> level_ptr, level_nodes;  // supernodes grouped by tree level
+  std::vector<int64_t> diag_index;  // panel slot of each diagonal entry (permuted column)
+  // off-diagonal Gram scatter map: segment t sums pairs seg_ptr[t]..seg_ptr[t+1]
+  // into panel slot seg_target[t]; pair k contributes w[col] * a[pa] * a[pb]
+  std::vector<int64_t> seg_target, seg_ptr;
+  std::vector<int> pair_col;
+  std::vector<int64_t> pair_a, pair_b;
+  int64_t nnz_panels = 0;  // stored panel entries (incl. upper parts of diagonal blocks)
+  int64_t nnz_L = 0;       // entries of L proper
+  int max_width = 0;
+  int64_t max_rows = 0;
+  double factor_flops = 0; // sum over columns of c_j^2
+};
+
+std::vector<int> minimum_degree_order(int m, int ncols, const int64_t* colptr,
+                                      const int* rowind, double dense_factor);
+
+Plan build_plan(int m, const int64_t* a_colptr, const int* a_rowind, const int* core,
+                int ncore, const PlanOptions& opt);
+
+}  // namespace psg
+
+// opaque handle of the C-ABI (include/psipm_hip.h)
+struct psg_plan {
+  psg::Plan plan;
+};

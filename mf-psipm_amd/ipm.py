@@ -1,0 +1,660 @@
+"""Proximal-stabilised interior-point driver (host side, FP64 numpy).
+
+This is the caller of the KKT object; it stays on the host and talks to the
+object only through ``factorize(theta, delta)``, ``solve(r1, r2)`` and the
+optional ``set_ipm_context`` / ``observe`` hooks and counters, exactly as the
+reference driver does.  Restated from psipm_active_pool.py:
+
+* ``step_to_boundary``   -- :1831-1837
+* ``box_floors`` / ``sync_box`` -- :1845-1872
+* ``Solution``           -- :1875-1941 (fields used on this path)
+* ``prox_eval``          -- :1947-2271 (pc = 1 plain, 2 Mehrotra, 3 Gondzio MCC)
+* ``psipm_solve``        -- :2277-2788 (outer proximal loop, ``kkt_solver`` path)
+* ``starting_point``     -- :4086-4304 (IPX-style start)
+
+The vector kernels of this loop are also available on the GPU (``ipm_gpu``);
+this module is the reference-faithful host version.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from .lp import StandardLP
+
+EPS = float(np.finfo(np.float64).eps)
+POS_FLOOR = 8.0 * EPS
+
+
+class KKTSolveError(RuntimeError):
+    """The KKT system could not be solved accurately enough (PS:553)."""
+
+
+def step_to_boundary(v: np.ndarray, dv: np.ndarray, idx) -> float:
+    """Largest alpha in (0, 1] with ``v + alpha dv >= 0`` on ``idx`` (PS:1831-1837)."""
+    vv, dd = v[idx], dv[idx]
+    neg = dd < 0.0
+    if not neg.any():
+        return 1.0
+    return float(min(1.0, np.min(-vv[neg] / dd[neg])))
+
+
+def _index(mask: np.ndarray):
+    return slice(None) if mask.all() else np.flatnonzero(mask)
+
+
+def box_floors(u: np.ndarray, ibox):
+    """Smallest resolvable distances from the two bounds of boxed columns (PS:1845-1857)."""
+    ub = np.asarray(u[ibox], dtype=float)
+    quarter = 0.25 * ub
+    return np.minimum(POS_FLOOR, quarter), np.minimum(POS_FLOOR * np.maximum(1.0, ub), quarter)
+
+
+def sync_box(x, s, u, ibox, x_floor, s_floor) -> None:
+    """Clip ``x`` into the box and reset ``s = u - x`` above its floor (PS:1860-1872)."""
+    x[ibox] = np.clip(x[ibox], x_floor, u[ibox] - s_floor)
+    s[ibox] = np.maximum(u[ibox] - x[ibox], s_floor)
+
+
+@dataclass
+class Solution:
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    w: np.ndarray
+    status: str = "unknown"
+    obj: float = np.nan
+    outer_iter: int = 0
+    inner_iter: int = 0
+    time: float = 0.0
+    t_fact: float = 0.0
+    t_solve: float = 0.0
+    res_p: float = np.nan
+    res_d: float = np.nan
+    mu: float = np.nan
+    gap: float = np.nan
+    history: list = field(default_factory=list)
+    n_not_pd: int = 0
+    n_shifted: int = 0
+    n_refine: int = 0
+    low_rank_columns: int = 0
+    low_rank_candidate_edges: int = 0
+    core_min: int = 0
+    core_max: int = 0
+    core_mean: float = 0.0
+    pcg_iterations: int = 0
+    pcg_max_iterations: int = 0
+    pcg_failures: int = 0
+    full_kkt_fallbacks: int = 0
+    preconditioner_refreshes: int = 0
+    preconditioner_factorizations: int = 0
+    preconditioner_retained_mean: float = 0.0
+    preconditioner_symbolic_analyses: int = 0
+    preconditioner_retries: int = 0
+    preconditioner_full: bool = False
+    preconditioner_condition_max: float = 1.0
+    preconditioner_spectral_error_max: float = 0.0
+    preconditioner_lag_bound_max: float = 1.0
+    preconditioner_core_swaps: int = 0
+    residual_replacements: int = 0
+    residual_checks: int = 0
+    krylov_stagnations: int = 0
+    minres_fallbacks: int = 0
+    minres_iterations: int = 0
+    krylov_breakdowns: int = 0
+    krylov_iteration_limits: int = 0
+
+
+def _regularise(reg_extra: float, tol: float) -> float:
+    """The ``reg_extra`` retry ladder of prox_eval (PS:2037)."""
+    return min(0.5, max(max(reg_extra, 1e-12) * 10.0, tol))
+
+
+def prox_eval(
+    c, A, At, b, u, ipos, ibox, npos_box, xk, yk, zk, wk, rho, delta, tol, maxit, pc,
+    kkt, printlevel=0, deadline=np.inf, nbox=0, tau=0.995,
+):
+    """Approximately solve one proximal sub-problem with a primal-dual IPM.
+
+    min c'v + rho/2 |v - xk|^2 + delta/2 |y|^2  s.t.  A v + delta (y - yk) = b,
+    0 <= v <= u.  Returns (x, y, z, w, info) with info status 0 max-iterations,
+    1 converged, 2 ill-conditioned, 3 time limit (PS:1947-2271).
+    """
+    n = A.shape[1]
+    m = A.shape[0]
+    x, y, z, w = xk.copy(), yk.copy(), zk.copy(), wk.copy()
+    x_floor, s_floor = box_floors(u, ibox) if nbox else (0.0, 0.0)
+    s = np.ones(n)
+    if nbox:
+        sync_box(x, s, u, ibox, x_floor, s_floor)
+
+    sigma_min, sigma_max = 0.05, 0.95
+    alpha_x = alpha_z = 0.0
+    reg_extra = 0.0
+    retry, max_tries = 0, 50
+    stalled = 0
+    best_res_n = np.inf
+    iters = 0
+    status = 0
+    mu = (x[ipos] @ z[ipos] + s[ibox] @ w[ibox]) / npos_box if npos_box else 0.0
+    rmz = np.zeros(n)
+    rmw = np.zeros(n)
+
+    def residuals():
+        rd = -(c - At @ y - z + w + rho * (x - xk))
+        prox_p = A @ x - b + delta * (y - yk)
+        g = c - At @ y + rho * (x - xk)
+        gp = g.copy()
+        gp[ipos] = x[ipos] - np.minimum(np.maximum(x[ipos] - g[ipos], 0.0), u[ipos])
+        return rd, -prox_p, np.linalg.norm(gp) + np.linalg.norm(prox_p)
+
+    res_d, res_p, res_n = residuals()
+
+    def newton(rd, rp, mz, mw):
+        r1 = rd.copy()
+        r1[ipos] += mz[ipos] / x[ipos]
+        r1[ibox] -= mw[ibox] / s[ibox]
+        try:
+            dx, dy = kkt.solve(r1, rp)
+        except (KKTSolveError, np.linalg.LinAlgError, FloatingPointError):
+            return None
+        if not (np.isfinite(dx).all() and np.isfinite(dy).all()):
+            return None
+        dz = np.zeros(n)
+        dw = np.zeros(n)
+        dz[ipos] = (mz[ipos] - z[ipos] * dx[ipos]) / x[ipos]
+        dw[ibox] = (mw[ibox] + w[ibox] * dx[ibox]) / s[ibox]
+        return dx, dy, dz, dw
+
+    def primal_step(d):
+        return tau * min(step_to_boundary(x, d, ipos), step_to_boundary(s, -d, ibox))
+
+    def dual_step(dz_, dw_):
+        return tau * min(step_to_boundary(z, dz_, ipos), step_to_boundary(w, dw_, ibox))
+
+    while iters < maxit:
+        prox_move = min(1.0, np.linalg.norm(x - xk) + np.linalg.norm(y - yk))
+        if res_n < 1.0e4 * tol * prox_move:
+            status = 1
+            break
+        if time.perf_counter() > deadline:
+            status = 3
+            break
+        iters += 1
+
+        theta_min = rho + reg_extra
+        theta = np.full(n, theta_min)
+        theta[ipos] += z[ipos] / x[ipos]
+        theta[ibox] += w[ibox] / s[ibox]
+        if not np.isfinite(theta).all():
+            theta = np.where(np.isfinite(theta), theta, theta_min / EPS**2)
+        np.clip(theta, theta_min, theta_min / EPS**2, out=theta)
+
+        failed = False
+        try:
+            if hasattr(kkt, "set_ipm_context"):
+                kkt.set_ipm_context(res_d=res_d, res_p=res_p, res_n=res_n, tol=tol)
+            if hasattr(kkt, "observe"):
+                kkt.observe(x, z, w, s, mu)
+            kkt.factorize(theta, delta + reg_extra)
+        except Exception:
+            failed = True
+        direction = None
+        if not failed:
+            zero_p, zero_d = np.zeros(m), np.zeros(n)
+            if pc == 1:
+                sigma = 0.5 if iters == 1 else max(1 - alpha_x, 1 - alpha_z) ** 5
+                sigma = min(max(sigma, sigma_min), sigma_max)
+                rmz[ipos] = sigma * mu - x[ipos] * z[ipos]
+                rmw[ibox] = sigma * mu - s[ibox] * w[ibox]
+                direction = newton(res_d, res_p, rmz, rmw)
+                sigma_used = sigma
+            else:
+                if pc == 2:
+                    rmz[ipos] = -x[ipos] * z[ipos]
+                    rmw[ibox] = -s[ibox] * w[ibox]
+                else:
+                    if iters > 1 or min(alpha_x, alpha_z) > 0.5:
+                        mu_target, weight = 0.1 * mu, 0.05
+                    else:
+                        mu_target, weight = 0.7 * mu, 0.1
+                    rmz[ipos] = -x[ipos] * z[ipos] + weight * mu_target
+                    rmw[ibox] = -s[ibox] * w[ibox] + weight * mu_target
+                direction = newton(res_d, res_p, rmz, rmw)
+                if direction is not None:
+                    dx, dy, dz, dw = direction
+                    alpha_x = primal_step(dx)
+                    alpha_z = dual_step(dz, dw)
+                    if pc == 2:
+                        direction, sigma_used = _mehrotra(
+                            x, s, z, w, ipos, ibox, npos_box, mu, direction,
+                            alpha_x, alpha_z, rmz, rmw, newton, zero_d, zero_p, tau,
+                        )
+                    else:
+                        sigma_used = mu_target / mu if mu > 0 else 0.0
+                        direction, alpha_x, alpha_z = _gondzio(
+                            x, s, z, w, ipos, ibox, mu_target, direction, alpha_x,
+                            alpha_z, rmz, rmw, newton, zero_d, zero_p, primal_step,
+                            dual_step,
+                        )
+        if direction is None:
+            if retry < max_tries:
+                iters -= 1
+                retry += 1
+                reg_extra = _regularise(reg_extra, tol)
+                continue
+            status = 2
+            break
+
+        dx, dy, dz, dw = direction
+        retry = 0
+        if npos_box:
+            alpha_x = primal_step(dx)
+            alpha_z = dual_step(dz, dw)
+        else:
+            alpha_x = alpha_z = 1.0
+        x += alpha_x * dx
+        y += alpha_z * dy
+        z += alpha_z * dz
+        w += alpha_z * dw
+        if nbox:
+            sync_box(x, s, u, ibox, x_floor, s_floor)
+        if reg_extra:
+            reg_extra = 0.0 if reg_extra <= 1e-12 else reg_extra * 0.1
+        if npos_box:
+            mu = (x[ipos] @ z[ipos] + s[ibox] @ w[ibox]) / npos_box
+        res_d, res_p, res_n = residuals()
+        if res_n < 0.999 * best_res_n:
+            best_res_n = res_n
+            stalled = 0
+        elif not np.isfinite(res_n):
+            stalled += 3
+        else:
+            stalled += 1
+        if printlevel >= 2:
+            print(
+                f"   {iters:4d}   {np.linalg.norm(res_p):8.2e}  {np.linalg.norm(res_d):8.2e}  "
+                f"{mu:8.2e}   {sigma_used:8.2e}  {alpha_x:6.4f}  {alpha_z:6.4f}"
+            )
+        if stalled >= 5:
+            break
+    return x, y, z, w, dict(iters=iters, status=status, stalled=stalled)
+
+
+def _mehrotra(x, s, z, w, ipos, ibox, npos_box, mu, direction, alpha_x, alpha_z,
+              rmz, rmw, newton, zero_d, zero_p, tau):
+    """Single Mehrotra corrector with the beta back-off ladder (PS:2108-2177)."""
+    dx, dy, dz, dw = direction
+    ds = -dx
+    cm = (x[ipos] + alpha_x * dx[ipos]) @ (z[ipos] + alpha_z * dz[ipos])
+    cm += (s[ibox] + alpha_x * ds[ibox]) @ (w[ibox] + alpha_z * dw[ibox])
+    mu_aff = cm / npos_box
+    mu_t = (mu_aff / mu) ** 2 * mu_aff if mu > 0 else mu_aff
+    rmz[ipos] = mu_t - dx[ipos] * dz[ipos]
+    rmw[ibox] = mu_t - ds[ibox] * dw[ibox]
+    corr = newton(zero_d, zero_p, rmz, rmw)
+    if corr is None:
+        return None, 0.0
+    best = None
+    for beta in (1.0, 0.5, 0.25, 0.125, 0.0625, 0.0):
+        dxb = dx + beta * corr[0]
+        dyb = dy + beta * corr[1]
+        dzb = dz + beta * corr[2]
+        dwb = dw + beta * corr[3]
+        axb = tau * min(step_to_boundary(x, dxb, ipos), step_to_boundary(s, -dxb, ibox))
+        azb = tau * min(step_to_boundary(z, dzb, ipos), step_to_boundary(w, dwb, ibox))
+        if axb <= 0.0 or azb <= 0.0:
+            continue
+        val = (x[ipos] + axb * dxb[ipos]) @ (z[ipos] + azb * dzb[ipos])
+        val += (s[ibox] - axb * dxb[ibox]) @ (w[ibox] + azb * dwb[ibox])
+        mub = float(val / npos_box)
+        if not np.isfinite(mub) or mub < 0.0:
+            continue
+        merit = mub / max(min(axb, azb), 1e-8)
+        if best is None or merit < best[0]:
+            best = (merit, beta, dxb, dyb, dzb, dwb)
+    if best is None:
+        return None, 0.0
+    _, beta, dxb, dyb, dzb, dwb = best
+    return (dxb, dyb, dzb, dwb), (beta * (mu_t / mu) if mu > 0 else 0.0)
+
+
+def _gondzio(x, s, z, w, ipos, ibox, mu_target, direction, alpha_x, alpha_z, rmz, rmw,
+             newton, zero_d, zero_p, primal_step, dual_step):
+    """Up to 10 multiple-centrality correctors (PS:2179-2216)."""
+    dx, dy, dz, dw = direction
+    bmin, bmax, delta_a = 1e-3, 1e3, 0.2
+    for _ in range(10):
+        ta_x = min(1.0, 1.5 * alpha_x + delta_a)
+        ta_z = min(1.0, 1.5 * alpha_z + delta_a)
+        for (p_, d_, target, idx) in (
+            (x + ta_x * dx, z + ta_z * dz, rmz, ipos),
+            (s - ta_x * dx, w + ta_z * dw, rmw, ibox),
+        ):
+            prod = p_[idx] * d_[idx]
+            corr = np.clip(prod, bmin * mu_target, bmax * mu_target) - prod
+            target[idx] = np.clip(corr, -bmax * mu_target, 2 * bmax * mu_target)
+        out = newton(zero_d, zero_p, rmz, rmw)
+        if out is None:
+            break
+        dxt, dyt, dzt, dwt = dx + out[0], dy + out[1], dz + out[2], dw + out[3]
+        a_xn = primal_step(dxt)
+        a_zn = dual_step(dzt, dwt)
+        if a_xn < 1.01 * alpha_x and a_zn < 1.01 * alpha_z:
+            break
+        alpha_x, alpha_z = a_xn, a_zn
+        dx, dy, dz, dw = dxt, dyt, dzt, dwt
+    return (dx, dy, dz, dw), alpha_x, alpha_z
+
+
+def starting_point(A, At, b, c, u, lp: StandardLP, ipos, ibox, solver,
+                   projection="legacy", bound_fraction=1e-3, linear_tolerance=None):
+    """IPX-style start projected onto the bounded form (PS:4122-4304)."""
+    m, n = A.shape
+    pos, box = lp.pos, lp.box
+    lower_only = pos & ~box
+    x, y = np.zeros(n), np.zeros(m)
+    saved = getattr(solver, "current_cg_tol", None)
+    relaxed = linear_tolerance is not None
+    if relaxed and saved is not None:
+        solver.current_cg_tol = max(
+            float(saved),
+            min(float(linear_tolerance), float(getattr(solver, "cg_tol_max", linear_tolerance))),
+        )
+    if relaxed:
+        try:
+            solver.factorize(np.ones(n), 1.0)
+        except Exception:
+            pass
+        else:
+            try:
+                x = At @ solver.solve(np.zeros(n), b)[1]
+            except Exception:
+                x = np.zeros(n)
+            try:
+                y = solver.solve(np.zeros(n), A @ c)[1]
+            except Exception:
+                y = np.zeros(m)
+    else:
+        try:
+            solver.factorize(np.ones(n), 1.0)
+            x = At @ solver.solve(np.zeros(n), b)[1]
+            y = solver.solve(np.zeros(n), A @ c)[1]
+        except Exception:
+            x, y = np.zeros(n), np.zeros(m)
+    if relaxed and saved is not None:
+        solver.current_cg_tol = saved
+    if not np.isfinite(x).all():
+        x = np.zeros(n)
+    if not np.isfinite(y).all():
+        y = np.zeros(m)
+    least_norm = x.copy()
+
+    xl = np.zeros(n)
+    xu = np.zeros(n)
+    xl[pos] = x[pos]
+    xu[box] = u[box] - x[box]
+    infeas = 0.0
+    if pos.any():
+        infeas = max(infeas, -float(np.min(xl[pos], initial=0.0)))
+    if box.any():
+        infeas = max(infeas, -float(np.min(xu[box], initial=0.0)))
+    xl[pos] += 1.0 + 1.5 * infeas
+    xu[box] += 1.0 + 1.5 * infeas
+
+    cnorm = float(np.linalg.norm(c))
+    if cnorm > 0.0:
+        r = c - At @ y
+        if float(np.linalg.norm(r)) < 0.05 * cnorm:
+            y *= 0.95
+            r = c - At @ y
+    else:
+        r = np.zeros(n)
+    z = np.zeros(n)
+    w = np.zeros(n)
+    if cnorm == 0.0:
+        z[pos] = 1.0
+        w[box] = 1.0
+    else:
+        z[lower_only] = r[lower_only]
+        z[box] = 0.5 * r[box]
+        w[box] = -0.5 * r[box]
+        zinf = 0.0
+        if pos.any():
+            zinf = max(zinf, -float(np.min(z[pos], initial=0.0)))
+        if box.any():
+            zinf = max(zinf, -float(np.min(w[box], initial=0.0)))
+        z[pos] += 1.0 + 1.5 * zinf
+        w[box] += 1.0 + 1.5 * zinf
+
+    xsum = 1.0 + float(np.sum(xl[pos])) + float(np.sum(xu[box]))
+    zsum = 1.0 + float(np.sum(z[pos])) + float(np.sum(w[box]))
+    mu0 = 1.0 + float(xl[pos] @ z[pos]) + float(xu[box] @ w[box])
+    xshift = 0.5 * mu0 / zsum if zsum > 0.0 else 0.0
+    zshift = 0.5 * mu0 / xsum if xsum > 0.0 else 0.0
+    if np.isfinite(xshift) and xshift > 0.0:
+        xl[pos] += xshift
+        xu[box] += xshift
+    if np.isfinite(zshift) and zshift > 0.0:
+        z[pos] += zshift
+        w[box] += zshift
+
+    if lower_only.any():
+        x[lower_only] = np.maximum(xl[lower_only], 1e-12)
+    if box.any():
+        denom = xl[box] + xu[box]
+        xb = np.where(denom > 0.0, u[box] * xl[box] / denom, 0.5 * u[box])
+        x_floor, s_floor = box_floors(u, ibox)
+        x[box] = np.clip(xb, x_floor, u[box] - s_floor)
+        clipped = least_norm.copy()
+        lo_m = np.maximum(x_floor, bound_fraction * u[box])
+        hi_m = np.maximum(s_floor, bound_fraction * u[box])
+        clipped[box] = np.clip(clipped[box], lo_m, u[box] - hi_m)
+        if lower_only.any():
+            clipped[lower_only] = np.maximum(clipped[lower_only], POS_FLOOR)
+        clipped = _repair_singletons(A, b, clipped, lower_only)
+        if projection == "adaptive" and _primal_score(A, b, clipped) < _primal_score(A, b, x):
+            x = clipped
+    return x, y, z, w
+
+
+def _repair_singletons(A, b, x, lower_only):
+    """Use unique singleton lower slacks to reduce the start residual (PS:4086-4107)."""
+    cols = np.flatnonzero(lower_only)
+    if not cols.size:
+        return x
+    cols = cols[np.diff(A.indptr)[cols] == 1]
+    if not cols.size:
+        return x
+    p = A.indptr[cols]
+    rows, vals = A.indices[p], A.data[p]
+    uniq, mult = np.unique(rows, return_counts=True)
+    keep = np.isin(rows, uniq[mult == 1])
+    cols, rows, vals = cols[keep], rows[keep], vals[keep]
+    if not cols.size:
+        return x
+    fixed = x[cols] + (b - A @ x)[rows] / vals
+    ok = np.isfinite(fixed) & (fixed > POS_FLOOR)
+    x[cols[ok]] = fixed[ok]
+    return x
+
+
+def _primal_score(A, b, x):
+    res = np.asarray(b - A @ x).ravel()
+    return max(
+        float(np.abs(res).max(initial=0.0)) / (1.0 + float(np.abs(b).max(initial=0.0))),
+        float(np.linalg.norm(res)) / (1.0 + float(np.linalg.norm(b))),
+    )
+
+
+def psipm_solve(
+    lp: StandardLP, kkt_solver, tol: float = 1e-6, maxit: int = 200, inner_maxit: int = 100,
+    pc: int = 3, rho: Optional[float] = None, delta: Optional[float] = None, rf: float = 1.0,
+    ppm_red: float = 0.5, stop: str = "relative", time_limit: float = np.inf,
+    printlevel: int = 0, compl: str = "gap", tau: float = 0.995, diverge_tol: float = 1e4,
+    initial=None, start_projection: str = "legacy", start_bound_fraction: float = 1e-3,
+    start_linear_tolerance: Optional[float] = None, **_ignored,
+) -> Solution:
+    """Outer proximal loop with an external KKT object (PS:2277-2788)."""
+    A = lp.A
+    At = A.T.tocsr()
+    c, b, u = lp.c, lp.b, lp.u
+    m, n = A.shape
+    ipos = _index(lp.pos)
+    ibox = _index(lp.box)
+    npos, nbox = int(lp.pos.sum()), int(lp.box.sum())
+    npos_box = npos + nbox
+    if start_projection not in {"adaptive", "legacy"}:
+        raise ValueError("start_projection must be 'adaptive' or 'legacy'")
+    normA = float(abs(A).sum(axis=1).max()) if A.nnz else 1.0
+    if rho is None:
+        rho = rf * max(tol / max(normA, 1e-16), 1e-8)
+    if delta is None:
+        delta = rho
+    sc = max(1.0, normA, float(np.abs(b).sum()), float(np.abs(c).sum()))
+    dp = 1.0 + float(np.abs(b).max(initial=0.0))
+    dd = 1.0 + float(np.abs(c).max(initial=0.0))
+    solver = kkt_solver
+    if getattr(solver, "A", A).shape != A.shape:
+        raise ValueError("external KKT solver dimensions do not match the LP")
+    low_rank = getattr(solver, "low_rank_columns", np.empty(0, dtype=np.int64))
+    edges = sum(int(q) * (int(q) - 1) // 2 for q in np.diff(A.indptr)[low_rank]) if low_rank.size else 0
+    x_floor, s_floor = box_floors(u, ibox) if nbox else (0.0, 0.0)
+
+    tic = time.perf_counter()
+    deadline = tic + time_limit
+    if initial is None:
+        x, y, z, w = starting_point(
+            A, At, b, c, u, lp, ipos, ibox, solver, projection=start_projection,
+            bound_fraction=start_bound_fraction, linear_tolerance=start_linear_tolerance,
+        )
+    else:
+        x, y, z, w = (np.asarray(v, dtype=float).copy() for v in initial)
+        if x.size != n or y.size != m or z.size != n or w.size != n:
+            raise ValueError("warm-start dimensions do not match the LP")
+        lower = np.full(n, 1e-12)
+        if nbox:
+            bx, bs = box_floors(u, ibox)
+            lower[ibox] = bx
+            x[ibox] = np.minimum(np.maximum(x[ibox], bx), u[ibox] - bs)
+        x[lp.pos] = np.maximum(x[lp.pos], lower[lp.pos])
+        z[lp.pos] = np.maximum(z[lp.pos], 1e-12)
+        w[lp.box] = np.maximum(w[lp.box], 1e-12)
+        x[~lp.pos] = np.asarray(initial[0], dtype=float)[~lp.pos]
+        z[~lp.pos] = 0.0
+        w[~lp.box] = 0.0
+
+    xk, yk, zk, wk = x.copy(), y.copy(), z.copy(), w.copy()
+    best_mu = np.inf
+    sol = Solution(x=x, y=y, z=z, w=w)
+    inner_total = 0
+    it = 0
+    for it in range(1, maxit + 1):
+        rp = b - A @ x
+        rd = c - At @ y - z + w
+        s = np.ones(n)
+        if nbox:
+            sync_box(x, s, u, ibox, x_floor, s_floor)
+        mu = (x[ipos] @ z[ipos] + s[ibox] @ w[ibox]) / npos_box if npos_box else 0.0
+        nrp1, nrpi = float(np.abs(rp).sum()), float(np.abs(rp).max(initial=0.0))
+        nrdi = float(np.abs(rd).max(initial=0.0))
+        comp = np.max(np.minimum(np.minimum(np.abs(x[ipos] * z[ipos]), np.abs(z[ipos])), np.abs(x[ipos])), initial=0.0)
+        if nbox:
+            comp = max(comp, np.max(np.minimum(np.minimum(np.abs(s[ibox] * w[ibox]), np.abs(w[ibox])), np.abs(s[ibox])), initial=0.0))
+        cx = float(c @ x)
+        gap = abs(cx - (float(b @ y) - float(w[ibox] @ u[ibox]))) / (1.0 + abs(cx + lp.obj_const))
+        sol.history.append(dict(outer=it, rp=nrpi / dp, rd=nrdi / dd, mu=mu, gap=gap, inner=inner_total,
+                                t=time.perf_counter() - tic))
+        if printlevel:
+            print(f"  PPM {it:3d} | rp {nrpi / dp:8.2e}  rd {nrdi / dd:8.2e}  mu {mu:8.2e}  "
+                  f"gap {gap:8.2e}  inner {inner_total:4d}  t {time.perf_counter() - tic:6.1f}s")
+        if stop == "psipm":
+            done = nrp1 < tol * sc and nrdi < tol * sc and comp < tol
+        else:
+            ok_c = (mu < tol) if compl == "mu" else (gap < tol) if compl == "gap" else (mu < tol or gap < tol)
+            done = nrpi / dp < tol and nrdi / dd < tol and ok_c
+        if done:
+            sol.status = "optimal"
+            break
+        if npos_box and mu > 0.0:
+            if mu < best_mu:
+                best_mu = mu
+            elif best_mu > 0.0 and mu > diverge_tol * best_mu:
+                sol.status = "diverged"
+                break
+        if time.perf_counter() > deadline:
+            sol.status = "time-limit"
+            break
+        x, y, z, w, info = prox_eval(
+            c, A, At, b, u, ipos, ibox, npos_box, xk, yk, zk, wk, rho, delta,
+            ppm_red**it, inner_maxit, pc, solver, printlevel, deadline, nbox=nbox, tau=tau,
+        )
+        inner_total += info["iters"]
+        if info["status"] == 2:
+            sol.status = "ill-conditioned"
+            break
+        xk, yk, zk, wk = x.copy(), y.copy(), z.copy(), w.copy()
+        if info["status"] == 3:
+            sol.status = "time-limit"
+            break
+    else:
+        sol.status = "max-iterations"
+
+    sol.x, sol.y, sol.z, sol.w = x, y, z, w
+    sol.outer_iter = it
+    sol.inner_iter = inner_total
+    sol.time = time.perf_counter() - tic
+    sol.t_fact, sol.t_solve = solver.t_fact, solver.t_solve
+    sol.res_p, sol.res_d, sol.mu, sol.gap = nrpi / dp, nrdi / dd, mu, gap
+    sol.obj = float(c @ x) + lp.obj_const
+    _collect_counters(sol, solver, low_rank, edges)
+    if printlevel:
+        print(f"  -> {sol.status}: obj {sol.obj:.10e}, {sol.outer_iter} outer / {sol.inner_iter} inner it, "
+              f"{sol.time:.1f}s (fact {sol.t_fact:.1f}s, solve {sol.t_solve:.1f}s), "
+              f"PCG {sol.pcg_iterations} total")
+    return sol
+
+
+def _finite_max(values, default):
+    arr = np.asarray(values, dtype=float)
+    arr = arr[np.isfinite(arr)]
+    return float(arr.max()) if arr.size else default
+
+
+def _collect_counters(sol: Solution, solver, low_rank, edges) -> None:
+    """Copy the KKT object's counters into the solution (PS:2680-2736)."""
+    sol.n_not_pd = getattr(solver, "stats", {}).get("not_pd", 0)
+    sol.n_shifted = getattr(solver, "n_shifted", 0)
+    sol.n_refine = getattr(solver, "n_refine", 0)
+    sol.low_rank_columns = int(np.asarray(low_rank).size)
+    sol.low_rank_candidate_edges = int(edges)
+    sizes = getattr(solver, "core_sizes", [])
+    if sizes:
+        sol.core_min, sol.core_max, sol.core_mean = min(sizes), max(sizes), float(np.mean(sizes))
+    iters = getattr(solver, "cg_iterations", [])
+    if iters:
+        sol.pcg_iterations, sol.pcg_max_iterations = int(sum(iters)), int(max(iters))
+    sol.pcg_failures = getattr(solver, "cg_failures", 0)
+    sol.full_kkt_fallbacks = getattr(solver, "full_fallbacks", 0)
+    sol.preconditioner_refreshes = getattr(solver, "refreshes", 0)
+    sol.preconditioner_factorizations = getattr(solver, "n_fact", 0)
+    retained = getattr(solver, "retained_fractions", [])
+    if retained:
+        sol.preconditioner_retained_mean = float(np.mean(retained))
+    sol.preconditioner_symbolic_analyses = getattr(solver, "n_symbolic", 0)
+    sol.preconditioner_retries = getattr(solver, "pcg_retries", 0)
+    sol.preconditioner_full = getattr(solver, "use_full_preconditioner", False)
+    sol.preconditioner_condition_max = _finite_max(getattr(solver, "condition_estimates", []), 1.0)
+    sol.preconditioner_spectral_error_max = _finite_max(getattr(solver, "spectral_error_estimates", []), 0.0)
+    sol.preconditioner_lag_bound_max = _finite_max(getattr(solver, "lag_spectral_bounds", []), 1.0)
+    sol.preconditioner_core_swaps = getattr(solver, "core_swaps", 0)
+    sol.residual_replacements = getattr(solver, "residual_replacements", 0)
+    sol.residual_checks = getattr(solver, "residual_checks", 0)
+    sol.krylov_stagnations = getattr(solver, "stagnation_events", 0)
+    sol.minres_fallbacks = getattr(solver, "minres_fallbacks", 0)
+    sol.minres_iterations = getattr(solver, "minres_iterations", 0)
+    sol.krylov_breakdowns = getattr(solver, "krylov_breakdowns", 0)
+    sol.krylov_iteration_limits = getattr(solver, "krylov_iteration_limits", 0)

@@ -1,0 +1,631 @@
+"""GPU spectral stable active-column KKT object -- the drop-in for the hot path.
+
+``HipSpectralStableColumnPreconditioner`` has the interface of the
+reference's ``SpectralStableColumnPreconditioner``
+(psipm_spectral_preconditioner.py:143-752, built on
+psipm_stable_preconditioner.py:113-517 and psipm_active_pool.py:720-996):
+
+* ``factorize(theta, delta)`` / ``solve(r1, r2) -> (dx, dy)`` plus the
+  ``set_ipm_context`` / ``observe`` hooks and every counter the driver reads
+  (psipm_active_pool.py:2680-2736), so ``psipm_solve`` runs unchanged;
+* host side (as the reference's design keeps it): column-selection rules,
+  matching backbone, factorise/refresh decisions, shift ladder, Krylov
+  bookkeeping, symbolic analysis (C++ in the library);
+* device side (libpsipm_hip.so, hand-written HIP for gfx950): H v, scores,
+  Gram assembly, supernodal Cholesky, triangular solves, the PCG loop, the
+  right-hand side and dx recovery.
+
+Errors: the library's negative status codes raise; a failed solve raises
+``ipm.KKTSolveError`` exactly where the reference raises it.  There is no
+CPU fallback: without the built library this module fails to construct.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import scipy.linalg as la
+import scipy.sparse as sp
+import scipy.sparse.csgraph as csgraph
+import scipy.sparse.linalg as spla
+
+from . import _lib
+from .ipm import KKTSolveError
+from .symbolic import SymbolicPlan
+
+SHIFTS = (0.0, 1e-14, 1e-12, 1e-10, 1e-8, 1e-6, 1e-4, 1e-2)  # PS:765
+
+
+def low_rank_columns(A: sp.csc_matrix, max_columns=8, min_nnz=1024, clique_ratio=2.0):
+    """Columns better kept as exact Woodbury updates (PS:691-717)."""
+    if max_columns <= 0 or A.shape[1] == 0:
+        return np.empty(0, dtype=np.int64)
+    q = np.diff(A.indptr).astype(float)
+    edges = 0.5 * q * np.maximum(q - 1.0, 0.0)
+    scale = max(float(A.nnz), float(A.shape[0]), 1.0)
+    cand = np.flatnonzero((q >= max(1, int(min_nnz))) & (edges >= float(clique_ratio) * scale))
+    if cand.size > max_columns:
+        cand = cand[np.argpartition(edges[cand], -max_columns)[-max_columns:]]
+    return np.sort(cand.astype(np.int64))
+
+
+def top_k(score: np.ndarray, count: int) -> np.ndarray:
+    """Largest finite scores ordered by (score desc, index asc) (SAP:97-110)."""
+    count = min(max(0, int(count)), score.size)
+    if count == 0:
+        return np.empty(0, dtype=np.int64)
+    finite = np.flatnonzero(np.isfinite(score))
+    if finite.size > count:
+        finite = finite[np.argpartition(score[finite], -count)[-count:]]
+    return finite[np.lexsort((finite, -score[finite]))].astype(np.int64)
+
+
+def ritz_estimate(alphas, betas):
+    """Extreme eigenvalues of the CG Lanczos tridiagonal (SSP:119-140)."""
+    k = len(alphas)
+    if k == 0:
+        return 1.0, 1.0, 1.0
+    a = np.asarray(alphas, dtype=float)
+    diag = np.empty(k)
+    diag[0] = 1.0 / a[0]
+    if k > 1:
+        b = np.asarray(betas[: k - 1], dtype=float)
+        diag[1:] = 1.0 / a[1:] + b / a[:-1]
+        ev = la.eigvalsh_tridiagonal(diag, np.sqrt(np.maximum(b, 0.0)) / a[:-1], check_finite=False)
+    else:
+        ev = diag
+    lo, hi = float(ev[0]), float(ev[-1])
+    if not np.isfinite(lo) or lo <= 0.0:
+        return lo, hi, np.inf
+    return lo, hi, hi / lo
+
+
+@dataclass
+class KrylovResult:
+    """Field-for-field the reference's KrylovResult (SSP:105-116)."""
+
+    solution: Optional[np.ndarray]
+    info: int
+    iterations: int
+    relative_residual: float
+    condition_estimate: float
+    spectral_error_estimate: float
+    residual: Optional[np.ndarray]
+    replacements: int
+    residual_checks: int
+    stagnated: bool
+
+
+class HipSpectralStableColumnPreconditioner:
+    """Spectral stable active-column Cholesky PCG on one MI355X."""
+
+    def __init__(
+        self, A: sp.csc_matrix, protected: Optional[np.ndarray] = None, precision: str = "fp64",
+        core_factor=1.0, max_core_factor=3.0, retention=0.9, factor_max_age=2, core_max_age=12,
+        refresh_cg=30, fill_weight=0.2, cg_tol=1e-7, cg_tol_max=1e-2, forcing_factor=0.1,
+        cg_maxit=200, retry_growth=1.35, full_preconditioner_row_threshold=1500,
+        full_preconditioner_clique_limit=2e8, full_preconditioner_nnz_limit=10_000_000,
+        structural_backbone=True, weighted_backbone=False, residual_score_weight=0.0,
+        swap_patience=2, condition_limit=1e12, lag_spectral_ratio=1e8, residual_replacement=100,
+        residual_gap_tolerance=0.1, stagnation_window=3, stagnation_reduction=0.8,
+        corrector_tightening=1.0, minres_fallback=True, minres_maxit=400,
+        minres_max_dimension=200_000, fail_tol=1e-6, low_rank_max_columns=8,
+        low_rank_min_nnz=1024, low_rank_clique_ratio=2.0, ordering=1, max_width=64,
+        dense_factor=10.0, max_factor_entries: Optional[float] = None, **_ignored,
+    ):
+        if core_factor <= 0.0:
+            raise ValueError("core_factor must be positive")
+        if max_core_factor < core_factor:
+            raise ValueError("max_core_factor must be at least core_factor")
+        if not 0.0 <= retention <= 1.0:
+            raise ValueError("retention must lie in [0, 1]")
+        if fill_weight < 0.0:
+            raise ValueError("fill_weight must be nonnegative")
+        if retry_growth <= 1.0:
+            raise ValueError("retry_growth must exceed one")
+        if not 0.0 < cg_tol <= cg_tol_max < 1.0:
+            raise ValueError("require 0 < cg_tol <= cg_tol_max < 1")
+        if forcing_factor <= 0.0:
+            raise ValueError("forcing_factor must be positive")
+        if weighted_backbone:
+            raise NotImplementedError("the weighted matching backbone is not part of this build")
+        self._lib = _lib.load()
+        self.A = A.tocsc()
+        self.A.sum_duplicates()
+        self.A.sort_indices()
+        self.m, self.n = self.A.shape
+        self.precision = precision
+        self.fail_tol = float(fail_tol)
+        self.low_rank_columns = low_rank_columns(self.A, low_rank_max_columns, low_rank_min_nnz,
+                                                 low_rank_clique_ratio)
+        if self.low_rank_columns.size:
+            raise NotImplementedError("Woodbury low-rank columns are not supported on the GPU path yet")
+        mask = np.ones(self.n, dtype=bool)
+        mask[self.low_rank_columns] = False
+        self.factor_columns = np.flatnonzero(mask)
+        counts = np.diff(self.A.indptr).astype(np.int64)[self.factor_columns]
+        coupling = np.flatnonzero(counts >= 2)
+        self.candidates = self.factor_columns[coupling]
+        self._candidates32 = np.ascontiguousarray(self.candidates, dtype=np.int32)
+        q = counts[coupling].astype(float)
+        self.clique_burden = 0.5 * q * np.maximum(q - 1.0, 0.0)
+        self.use_full_preconditioner = bool(
+            self.m <= max(0, int(full_preconditioner_row_threshold))
+            and self.A.nnz <= max(0, int(full_preconditioner_nnz_limit))
+            and float(self.clique_burden.sum()) <= float(full_preconditioner_clique_limit))
+        pmask = np.zeros(self.n, dtype=bool)
+        if protected is not None:
+            supplied = np.asarray(protected, dtype=bool)
+            if supplied.shape != (self.n,):
+                raise ValueError("protected mask has the wrong dimension")
+            pmask[:] = supplied
+        self.protected_mandatory = self.candidates[pmask[self.candidates]]
+        self.mandatory = self.protected_mandatory.copy()
+        self.matching_columns = np.empty(0, dtype=np.int64)
+        self.structural_backbone = bool(structural_backbone)
+        requested = max(1, int(np.ceil(float(core_factor) * self.m)))
+        maximum = max(requested, int(np.ceil(float(max_core_factor) * self.m)))
+        if self.use_full_preconditioner:
+            requested = maximum = self.candidates.size
+        self.core_target = min(self.candidates.size, max(requested, self.mandatory.size))
+        self.max_core_target = min(self.candidates.size, max(maximum, self.mandatory.size))
+        self.retention = float(retention)
+        self.factor_max_age = 0 if self.use_full_preconditioner else max(0, int(factor_max_age))
+        self.core_max_age = max(1, int(core_max_age))
+        self.refresh_cg = max(1, int(refresh_cg))
+        self.fill_weight = float(fill_weight)
+        self.cg_tol, self.cg_tol_max = float(cg_tol), float(cg_tol_max)
+        self.forcing_factor = float(forcing_factor)
+        self.current_cg_tol = self.cg_tol
+        self.cg_maxit = max(1, int(cg_maxit))
+        self.retry_growth = float(retry_growth)
+        self.residual_score_weight = max(0.0, float(residual_score_weight))
+        self.swap_patience = max(1, int(swap_patience))
+        self.condition_limit = max(1.0, float(condition_limit))
+        self.lag_spectral_ratio = max(1.0, float(lag_spectral_ratio))
+        self.residual_replacement = max(0, int(residual_replacement))
+        self.residual_gap_tolerance = max(np.sqrt(np.finfo(float).eps), float(residual_gap_tolerance))
+        self.stagnation_window = max(2, int(stagnation_window))
+        self.stagnation_reduction = float(stagnation_reduction)
+        self.corrector_tightening = min(1.0, max(0.01, float(corrector_tightening)))
+        self.minres_fallback = bool(minres_fallback)
+        self.minres_maxit = max(1, int(minres_maxit))
+        self.minres_max_dimension = max(0, int(minres_max_dimension))
+        self.ordering, self.max_width, self.dense_factor = int(ordering), int(max_width), float(dense_factor)
+        self.max_factor_entries = None if max_factor_entries is None else float(max_factor_entries)
+
+        self.core = np.empty(0, dtype=np.int64)
+        self._weights = None
+        self.theta = None
+        self.delta = 0.0
+        self.factor_age = self.factor_max_age
+        self.core_age = self.core_max_age
+        self._initialized = False
+        self._last_cg = self.refresh_cg
+        self._have_factor = False
+        self._plan = None
+        self._plan_key = None
+        self._shift = 0.0
+        self._enrichment = None
+        self._reference_weights = None
+        self._reference_delta = None
+        self._poor_streak = 0
+        self._last_condition = 1.0
+        self._last_spectral_error = 0.0
+        self._last_stagnated = False
+        self._rhs_index = 0
+        self._nonlinear = None
+        self._ew_tolerance = self.cg_tol_max
+        self.nnz_factor = 0
+        # counters read by the driver (PS:2680-2736)
+        self.n_fact = self.n_solve = 0
+        self.t_fact = self.t_solve = 0.0
+        self.n_shifted = self.n_refine = self.n_symbolic = 0
+        self.stats = {}
+        self.core_sizes, self.cg_iterations, self.cg_tolerances = [], [], []
+        self.cg_failures = self.pcg_retries = self.full_fallbacks = self.refreshes = 0
+        self.retained_fractions, self.preconditioner_nnz = [], []
+        self.condition_estimates, self.spectral_error_estimates, self.lag_spectral_bounds = [], [], []
+        self.residual_replacements = self.residual_checks = self.stagnation_events = 0
+        self.core_swaps = self.minres_fallbacks = self.minres_iterations = 0
+        self.krylov_breakdowns = self.krylov_iteration_limits = 0
+        # device-time accounting (HIP events)
+        self.device_pcg_ms = 0.0
+        self.device_factor_ms = 0.0
+        self.pcg_calls = 0
+
+        prec = {"fp64": _lib.PSG_FP64, "mixed": _lib.PSG_MIXED}[precision]
+        handle = C.c_void_p()
+        self._call("psg_create", self.m, self.n, np.ascontiguousarray(self.A.indptr, dtype=np.int64),
+                   np.ascontiguousarray(self.A.indices, dtype=np.int32),
+                   np.ascontiguousarray(self.A.data, dtype=np.float64), prec, C.byref(handle))
+        self._ctx = handle
+        self._At = None
+
+    # ------------------------------------------------------------------ utils
+    def _call(self, name, *args):
+        status = getattr(self._lib, name)(*args)
+        if status < 0:
+            raise _lib.LibraryError(name, status, self._lib.psg_last_error().decode())
+        return status
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.psg_destroy(self._ctx)
+            self._ctx = None
+        if getattr(self, "_plan", None) is not None:
+            self._plan.close()
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def At(self):
+        if self._At is None:
+            self._At = self.A.T.tocsr()
+        return self._At
+
+    # -------------------------------------------------------------- IPM hooks
+    def set_ipm_context(self, res_d, res_p, res_n, tol):
+        """Eisenstat-Walker forcing from nonlinear progress (SSP:230-250)."""
+        tiny = np.finfo(float).tiny
+        current = max(float(res_n), float(np.linalg.norm(res_d)), float(np.linalg.norm(res_p)), tiny)
+        prev = self._nonlinear
+        self._nonlinear = current
+        est = self.cg_tol_max if prev is None or not np.isfinite(prev) else 0.9 * (current / max(prev, tiny)) ** 1.5
+        self._ew_tolerance = min(self.cg_tol_max, max(self.cg_tol, float(tol) * 0.1, est))
+        self._rhs_index = 0
+
+    def observe(self, x, z, w, s, mu):
+        """mu-based forcing capped by the EW estimate (SSP:252-258)."""
+        nm = max(float(mu), 0.0) / (1.0 + max(float(mu), 0.0))
+        self.current_cg_tol = min(self.cg_tol_max, max(self.cg_tol, min(self.forcing_factor * nm, self._ew_tolerance)))
+
+    def _linear_tolerance(self):
+        tol = self.current_cg_tol
+        if self._rhs_index:
+            tol = max(self.cg_tol, self.corrector_tightening * tol)
+        self._rhs_index += 1
+        return float(tol)
+
+    # -------------------------------------------------------------- selection
+    def _structural_matching(self):
+        """Hopcroft-Karp backbone on the pattern of A_factor (SSP:307-326)."""
+        if not self.structural_backbone or self.use_full_preconditioner or not self.m or not self.factor_columns.size:
+            return np.empty(0, dtype=np.int64)
+        g = self.A[:, self.factor_columns].tocsr(copy=True)
+        g.data.fill(1.0)
+        local = csgraph.maximum_bipartite_matching(g, perm_type="column")
+        glob = self.factor_columns[local[local >= 0]]
+        pos = np.searchsorted(self.candidates, glob)
+        ok = pos < self.candidates.size
+        idx = np.flatnonzero(ok)
+        ok[idx] &= self.candidates[pos[idx]] == glob[idx]
+        return np.unique(glob[ok])
+
+    def _update_backbone(self):
+        if self.matching_columns.size:
+            return
+        match = self._structural_matching()
+        if match.size:
+            self.matching_columns = match
+            self.mandatory = np.union1d(self.protected_mandatory, match)
+            self.core_target = min(self.candidates.size, max(self.core_target, self.mandatory.size))
+            self.max_core_target = min(self.candidates.size, max(self.max_core_target, self.mandatory.size))
+
+    def _scores(self, weights, delta):
+        """Leverage scores on the GPU (SAP:293-309), residual enrichment on the host (SSP:367-386)."""
+        if not self.candidates.size:
+            return np.empty(0)
+        scores = np.empty(self.candidates.size)
+        self._call("psg_scores", self._ctx, self._candidates32, self.candidates.size, float(delta),
+                   self.fill_weight, scores)
+        v = self._enrichment
+        if self.residual_score_weight > 0.0 and v is not None and v.size == self.m:
+            nv = float(np.linalg.norm(v))
+            if np.isfinite(nv) and nv > 0.0:
+                v = v / nv
+                Ac = self.A[:, self.candidates]
+                prod = np.asarray(Ac.T @ v).ravel()
+                dg = np.asarray(Ac.multiply(Ac).T @ np.square(v)).ravel()
+                impact = weights[self.candidates] * np.abs(np.square(prod) - dg)
+                impact /= 1.0 + self.fill_weight * self.clique_burden
+                s_sc = float(np.max(scores, initial=0.0))
+                i_sc = float(np.max(impact, initial=0.0))
+                if i_sc > 0.0 and np.isfinite(i_sc):
+                    scores = scores + self.residual_score_weight * max(s_sc, 1.0) * (impact / i_sc)
+        return scores
+
+    def _refresh_core(self, weights, delta, stable):
+        """Mandatory + retained + top-score fill (SAP:311-371, SSP:388-394)."""
+        old = self.core.copy()
+        self._update_backbone()
+        if self.use_full_preconditioner:
+            self.core = self.candidates.copy()
+            self.retained_fractions.append(1.0)
+        else:
+            scores = self._scores(weights, delta)
+            target = min(self.core_target, self.candidates.size)
+            picked = []
+            mand_local = np.searchsorted(self.candidates, self.mandatory)
+            if self.mandatory.size:
+                picked.append(mand_local)
+            if stable and old.size and target:
+                old_local = np.searchsorted(self.candidates, old)
+                keep = min(old.size, max(0, int(np.floor(self.retention * target))))
+                if keep:
+                    picked.append(old_local[top_k(scores[old_local], keep)])
+            fixed = np.unique(np.concatenate(picked)) if picked else np.empty(0, dtype=np.int64)
+            if fixed.size > target:
+                if self.mandatory.size >= target:
+                    fixed = mand_local[:target]
+                else:
+                    optional = fixed[~np.isin(fixed, mand_local, assume_unique=True)]
+                    fixed = np.concatenate([mand_local, optional[top_k(scores[optional], target - self.mandatory.size)]])
+            remaining = target - fixed.size
+            if remaining > 0:
+                avail = scores.copy()
+                avail[fixed] = -np.inf
+                fixed = np.concatenate([fixed, top_k(avail, remaining)])
+            new = np.sort(self.candidates[np.unique(fixed)])
+            self.retained_fractions.append(
+                float(np.intersect1d(old, new, assume_unique=True).size) / max(1, old.size) if old.size else 1.0)
+            self.core = new
+            self._last_scores = scores
+        self.refreshes += 1
+        self.core_age = 0
+        self._initialized = True
+        if old.size and self.core.size:
+            self.core_swaps += max(0, int(old.size - np.intersect1d(old, self.core, assume_unique=True).size))
+
+    # ------------------------------------------------------------ factorisation
+    def _ensure_plan(self):
+        key = self.core.tobytes()
+        if self._plan is not None and key == self._plan_key:
+            return self._plan
+        if self._plan is not None:
+            self._plan.close()
+        self._plan = SymbolicPlan(self.A, self.core, ordering=self.ordering, max_width=self.max_width,
+                                  dense_factor=self.dense_factor, lib=self._lib)
+        self._plan_key = key
+        self.n_symbolic += 1
+        return self._plan
+
+    def _factor_numeric(self, theta, delta):
+        """Assemble, Jacobi-scale, shift ladder and numeric factor (PS:839-897)."""
+        t0 = time.perf_counter()
+        plan = self._ensure_plan()
+        start = 0
+        if self._shift > 0.0:
+            start = max(0, int(np.searchsorted(SHIFTS, self._shift, side="left")) - 1)
+        nnz = C.c_double(0.0)
+        for eta in SHIFTS[start:]:
+            status = self._call("psg_factor", self._ctx, plan.handle, float(eta), C.byref(nnz))
+            if status == 0:
+                self._shift = eta
+                if eta:
+                    self.n_shifted += 1
+                break
+            self.stats["not_pd"] = self.stats.get("not_pd", 0) + 1
+        else:
+            self._have_factor = False
+            self.t_fact += time.perf_counter() - t0
+            raise KKTSolveError("Cholesky failed even with a 1e-2 shift")
+        fms = C.c_double(0.0)
+        self._call("psg_last_timings", self._ctx, None, C.byref(fms))
+        self.device_factor_ms += fms.value
+        self._have_factor = True
+        self.nnz_factor = int(nnz.value)
+        self.theta = theta
+        self.delta = float(delta)
+        self.n_fact += 1
+        self.t_fact += time.perf_counter() - t0
+
+    def _lag_bound(self, weights, delta):
+        """Componentwise weight ratio since the last factor (SSP:396-410)."""
+        if self._reference_weights is None:
+            return np.inf
+        ratio = weights / np.maximum(self._reference_weights, np.finfo(float).tiny)
+        fin = ratio[np.isfinite(ratio) & (ratio > 0.0)]
+        if fin.size != ratio.size:
+            return np.inf
+        bound = max(float(fin.max()), 1.0 / float(fin.min()))
+        if self._reference_delta > 0.0 and delta > 0.0:
+            dr = delta / self._reference_delta
+            bound = max(bound, dr, 1.0 / dr)
+        elif delta != self._reference_delta:
+            return np.inf
+        return float(bound)
+
+    def _factor_current(self, theta, delta):
+        self._factor_numeric(theta, delta)
+        self.factor_age = 0
+        self._reference_weights = self._weights.copy()
+        self._reference_delta = float(delta)
+        self.preconditioner_nnz.append(self.nnz_factor)
+
+    def factorize(self, theta, delta):
+        """Refresh / refactor / lag decisions of SSP:420-467, on the GPU factor."""
+        theta = np.ascontiguousarray(theta, dtype=float)
+        self._weights = 1.0 / theta
+        self._call("psg_set_weights", self._ctx, theta, float(delta))
+        spectral_poor = self._last_condition >= self.condition_limit and self._last_cg >= max(10, int(0.8 * self.refresh_cg))
+        poor = self._last_cg >= self.refresh_cg or spectral_poor or self._last_stagnated
+        self._poor_streak = self._poor_streak + 1 if poor else 0
+        can_grow = self.core_target < self.max_core_target
+        swap_due = poor and not can_grow and self._poor_streak >= self.swap_patience
+        refresh = not self._initialized or self.core_age >= self.core_max_age or (poor and can_grow) or swap_due
+        if refresh:
+            self._refresh_core(self._weights, delta, stable=bool(self.core.size))
+            self._poor_streak = 0
+        lag = self._lag_bound(self._weights, float(delta))
+        self.lag_spectral_bounds.append(lag)
+        if (not self._have_factor or self.factor_age >= self.factor_max_age or refresh or poor
+                or lag > self.lag_spectral_ratio):
+            self._factor_current(theta, float(delta))
+        else:
+            self.theta = theta
+            self.delta = float(delta)
+            self.factor_age += 1
+        self.core_age += 1
+        self.core_sizes.append(int(self.core.size))
+
+    # ------------------------------------------------------------ operators
+    def _full_matvec(self, v):
+        """H v = A (W (A' v)) + delta v on the GPU (SAP:434-438)."""
+        out = np.empty(self.m)
+        self._call("psg_apply_H", self._ctx, np.ascontiguousarray(v, dtype=float), out)
+        return out
+
+    def _apply(self, r):
+        """z = P^-1 r on the GPU (PS:949-956; no Woodbury columns on this path)."""
+        out = np.empty(self.m)
+        self._call("psg_apply_P", self._ctx, np.ascontiguousarray(r, dtype=float), out)
+        return out
+
+    # ------------------------------------------------------------------ PCG
+    def _pcg(self, tol) -> KrylovResult:
+        """One PCG run on the device-resident rhs (SSP:469-581)."""
+        res = _lib.PsgPcgResult()
+        cap = self.cg_maxit + 2
+        alphas, betas, hist = np.empty(cap), np.empty(cap), np.empty(cap)
+        self._call("psg_pcg", self._ctx, float(tol), self.cg_maxit, self.residual_replacement,
+                   self.residual_gap_tolerance, C.byref(res), alphas, betas, hist)
+        pms = C.c_double(0.0)
+        self._call("psg_last_timings", self._ctx, C.byref(pms), None)
+        self.device_pcg_ms += pms.value
+        self.pcg_calls += 1
+        lo, hi, cond = ritz_estimate(alphas[: res.n_alpha], betas[: res.n_beta])
+        h = hist[: res.n_history]
+        wdw = self.stagnation_window
+        stagnated = h.size >= wdw and h[-1] > self.stagnation_reduction * h[-wdw]
+        residual = None
+        if self.residual_score_weight > 0.0:
+            residual = np.empty(self.m)
+            self._call("psg_pcg_residual", self._ctx, residual)
+        return KrylovResult(None, int(res.info), int(res.iterations), float(res.relative_residual),
+                            float(cond), float(max(abs(lo - 1.0), abs(hi - 1.0))), residual,
+                            int(res.replacements), int(res.residual_checks), bool(stagnated))
+
+    def _minres(self, r1, r2, tol):
+        """Augmented-system MINRES recovery (SSP:583-637); P^-1 applied on the GPU."""
+        size = self.n + self.m
+        count = [0]
+        A, At = self.A, self.At
+
+        def matvec(v):
+            dx, q = v[: self.n], v[self.n:]
+            return np.concatenate([self.theta * dx + At @ q, A @ dx - self.delta * q])
+
+        def prec(v):
+            return np.concatenate([v[: self.n] / self.theta, self._apply(v[self.n:])])
+
+        op = spla.LinearOperator((size, size), matvec=matvec, dtype=float)
+        pm = spla.LinearOperator((size, size), matvec=prec, dtype=float)
+        rhs = np.concatenate([r1, r2])
+        scale = max(1.0, float(np.linalg.norm(rhs)))
+        target = max(2.0 * float(tol), self.fail_tol)
+        sol = np.zeros_like(rhs)
+        for _ in range(3):
+            cr = rhs - matvec(sol)
+            rel = float(np.linalg.norm(cr)) / scale
+            if np.isfinite(rel) and rel <= target:
+                break
+            corr, info = spla.minres(op, cr, M=pm, rtol=max(np.finfo(float).eps, min(float(tol), 1e-8)),
+                                     maxiter=self.minres_maxit,
+                                     callback=lambda _: count.__setitem__(0, count[0] + 1))
+            sol += corr
+            if info < 0:
+                break
+        rel = float(np.linalg.norm(rhs - matvec(sol))) / scale
+        return sol[: self.n], -sol[self.n:], bool(np.isfinite(rel) and rel <= target), count[0], rel
+
+    def _record(self, res: KrylovResult):
+        self._last_cg = int(res.iterations)
+        self._last_condition = float(res.condition_estimate)
+        self._last_spectral_error = float(res.spectral_error_estimate)
+        self._last_stagnated = bool(res.stagnated)
+        if res.residual is not None:
+            self._enrichment = res.residual.copy()
+        self.condition_estimates.append(self._last_condition)
+        self.spectral_error_estimates.append(self._last_spectral_error)
+        self.residual_replacements += int(res.replacements)
+        self.residual_checks += int(res.residual_checks)
+        self.stagnation_events += int(res.stagnated)
+
+    def solve(self, r1, r2):
+        """(dx, dy) of the proximal Newton system with retry and fallback (SSP:651-752)."""
+        started = time.perf_counter()
+        fact_before = self.t_fact
+        r1 = np.ascontiguousarray(r1, dtype=float)
+        r2 = np.ascontiguousarray(r2, dtype=float)
+        self._call("psg_solve_begin", self._ctx, r1, r2)
+        tol = self._linear_tolerance()
+        res = self._pcg(tol)
+        total = res.iterations
+        failed = res.info != 0 or not np.isfinite(res.relative_residual) or res.relative_residual > tol
+        can_grow = self.core_target < self.max_core_target
+        if failed and (can_grow or res.info < 0 or res.stagnated) and self.candidates.size and not self.use_full_preconditioner:
+            self.pcg_retries += 1
+            if res.residual is not None:
+                self._enrichment = res.residual.copy()
+            if self.core_target < self.max_core_target:
+                self.core_target = min(self.max_core_target,
+                                       max(self.core_target + 1, int(np.ceil(self.retry_growth * self.core_target))))
+            self._refresh_core(self._weights, self.delta, stable=True)
+            self._factor_current(self.theta, self.delta)
+            self.core_sizes.append(int(self.core.size))
+            res = self._pcg(tol)
+            total += res.iterations
+            failed = res.info != 0 or not np.isfinite(res.relative_residual) or res.relative_residual > tol
+        if failed and res.relative_residual <= max(2.0 * tol, self.fail_tol):
+            failed = False
+        self.cg_tolerances.append(tol)
+        self.cg_iterations.append(total)
+        self._record(res)
+        if res.info < 0:
+            self.krylov_breakdowns += 1
+        elif res.info > 0:
+            self.krylov_iteration_limits += 1
+        if not failed:
+            dx, dy = np.empty(self.n), np.empty(self.m)
+            self._call("psg_solve_finish", self._ctx, dx, dy)
+        elif self.minres_fallback and self.n + self.m <= self.minres_max_dimension and (res.info < 0 or res.stagnated):
+            self.cg_failures += 1
+            self.minres_fallbacks += 1
+            dx, dy, ok, its, rel = self._minres(r1, r2, tol)
+            self.minres_iterations += its
+            self.full_fallbacks += 1
+            if not ok:
+                self.t_solve += max(0.0, time.perf_counter() - started - (self.t_fact - fact_before))
+                raise KKTSolveError(f"spectral PCG and augmented MINRES failed: pcg info={res.info}, "
+                                    f"pcg residual={res.relative_residual:.2e}, minres residual={rel:.2e}")
+        else:
+            self.cg_failures += 1
+            self.t_solve += max(0.0, time.perf_counter() - started - (self.t_fact - fact_before))
+            raise KKTSolveError(f"spectral stable-column PCG failed: info={res.info}, "
+                                f"residual={res.relative_residual:.2e}")
+        self.n_solve += 1
+        self.t_solve += max(0.0, time.perf_counter() - started - (self.t_fact - fact_before))
+        return dx, dy
+
+
+def solve_standard(lp, precision="fp64", **kwargs):
+    """Build the GPU object and run the unchanged driver (SSP:755-840)."""
+    from .ipm import psipm_solve
+
+    solver_keys = set(HipSpectralStableColumnPreconditioner.__init__.__code__.co_varnames)
+    driver_keys = set(psipm_solve.__code__.co_varnames)
+    opts = {k: v for k, v in kwargs.items() if k in solver_keys and k not in driver_keys}
+    solver = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, precision=precision, **opts)
+    try:
+        return psipm_solve(lp, kkt_solver=solver, **{k: v for k, v in kwargs.items() if k in driver_keys}), solver
+    finally:
+        pass

@@ -1,0 +1,73 @@
+"""Host symbolic analysis of the active-column preconditioner (C++ in the library).
+
+Python face of ``psg_plan_*``: ordering, elimination tree, supernode
+partition and the Gram scatter map of ``M = A_C W_C A_C' + diag``.  Replaces
+CHOLMOD's analyze step (psipm_active_pool.py:668-671) and its pattern-reuse
+test (psipm_active_pool.py:842-846): a plan depends only on the active
+column set, so it is rebuilt only when the core changes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+
+INFO_KEYS = ("m", "nsuper", "nlevels", "nnz_panels", "nnz_L", "npairs", "nsegments",
+             "max_width", "max_rows", "factor_flops", "row_structure")
+
+
+class SymbolicPlan:
+    """Supernodal plan of P M P' for the columns ``core`` of ``A``."""
+
+    def __init__(self, A: sp.csc_matrix, core: np.ndarray, ordering: int = 1,
+                 max_width: int = 64, dense_factor: float = 10.0, lib=None):
+        self._lib = lib or _lib.load()
+        A = A.tocsc()
+        self.m, self.n = A.shape
+        self.core = np.ascontiguousarray(core, dtype=np.int32)
+        colptr = np.ascontiguousarray(A.indptr, dtype=np.int64)
+        rowind = np.ascontiguousarray(A.indices, dtype=np.int32)
+        handle = C.c_void_p()
+        status = self._lib.psg_plan_create(
+            self.m, self.n, colptr, rowind, self.core, self.core.size, int(ordering),
+            int(max_width), float(dense_factor), C.byref(handle))
+        if status < 0:
+            raise _lib.LibraryError("psg_plan_create", status, self._lib.psg_last_error().decode())
+        self.handle = handle
+        info = np.zeros(12)
+        self._lib.psg_plan_info(self.handle, info)
+        self.info = {k: int(v) if k != "factor_flops" else float(v) for k, v in zip(INFO_KEYS, info)}
+
+    def export(self) -> dict:
+        i = self.info
+        ns, m = i["nsuper"], i["m"]
+        out = dict(
+            perm=np.empty(m, np.int32), sfirst=np.empty(ns + 1, np.int32),
+            rptr=np.empty(ns + 1, np.int64), rows=np.empty(i["row_structure"], np.int32),
+            poff=np.empty(ns + 1, np.int64), sparent=np.empty(ns, np.int32),
+            level_ptr=np.empty(i["nlevels"] + 1, np.int32), level_nodes=np.empty(ns, np.int32),
+            diag_index=np.empty(m, np.int64), seg_target=np.empty(i["nsegments"], np.int64),
+            seg_ptr=np.empty(i["nsegments"] + 1, np.int64), pair_col=np.empty(i["npairs"], np.int32),
+            pair_a=np.empty(i["npairs"], np.int64), pair_b=np.empty(i["npairs"], np.int64),
+        )
+        status = self._lib.psg_plan_export(
+            self.handle, out["perm"], out["sfirst"], out["rptr"], out["rows"], out["poff"],
+            out["sparent"], out["level_ptr"], out["level_nodes"], out["diag_index"],
+            out["seg_target"], out["seg_ptr"], out["pair_col"], out["pair_a"], out["pair_b"])
+        if status < 0:
+            raise _lib.LibraryError("psg_plan_export", status, self._lib.psg_last_error().decode())
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.psg_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

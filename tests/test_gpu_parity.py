@@ -1,0 +1,207 @@
+"""GPU parity: every hot-path entry point of libpsipm_hip against the oracle.
+
+Each test prints the case it runs before touching the GPU.  Sizes keep the
+CPU oracle to seconds.  Tolerances (FP64 path): reduction-order only for
+H v and scores (1e-12 relative); 1e-9 relative for P^-1 r on matrices of
+condition <= 1e6; PCG solutions to the requested tolerance.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.linalg as la
+import scipy.sparse as sp
+
+import mf_psipm_amd as P
+from mf_psipm_amd import _lib
+from mf_psipm_amd.preconditioner import HipSpectralStableColumnPreconditioner
+from mf_psipm_amd.symbolic import SymbolicPlan
+from oracle.ssp_cpu import SpectralStableColumnPreconditionerCPU
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def _lp(m, n, k, seed):
+    raw, *_ = P.generators.random_lp(m, n, k, seed)
+    lp = P.lp.to_standard_form(raw)
+    P.lp.scale_problem(lp)
+    return raw, lp
+
+
+def _theta(n, seed, spread=6.0):
+    rng = np.random.default_rng(seed)
+    return 10.0 ** rng.uniform(-spread, spread, n)
+
+
+@pytest.mark.parametrize("m,n,k", [(500, 1500, 5), (2000, 6000, 5), (3000, 9000, 8)])
+def test_apply_H_matches_scipy(gpu, m, n, k):
+    _say(f"apply_H m={m} n={n} k={k}")
+    _, lp = _lp(m, n, k, 1)
+    A = lp.A
+    obj = HipSpectralStableColumnPreconditioner(A, protected=~lp.pos)
+    theta = _theta(n, 2)
+    delta = 1e-4
+    obj._call("psg_set_weights", obj._ctx, theta, delta)
+    v = np.random.default_rng(3).standard_normal(m)
+    got = obj._full_matvec(v)
+    ref = A @ ((1.0 / theta) * (A.T @ v)) + delta * v
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    obj.close()
+
+
+def test_apply_H_grid_network(gpu):
+    _say("apply_H grid 30x30")
+    lp = P.generators.grid_min_cost_flow(30, 30, 20, 5)
+    lp, _ = P.lp.ground_network(lp)
+    obj = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, structural_backbone=False)
+    theta = _theta(lp.A.shape[1], 4)
+    obj._call("psg_set_weights", obj._ctx, theta, 1e-6)
+    v = np.random.default_rng(5).standard_normal(lp.A.shape[0])
+    ref = lp.A @ ((1.0 / theta) * (lp.A.T @ v)) + 1e-6 * v
+    assert np.allclose(obj._full_matvec(v), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    obj.close()
+
+
+def test_scores_and_diag_terms_match_oracle(gpu):
+    _say("scores m=1600")
+    _, lp = _lp(1600, 4800, 5, 7)
+    theta = _theta(4800, 8)
+    gpu_obj = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos)
+    cpu_obj = SpectralStableColumnPreconditionerCPU(lp.A, protected=~lp.pos)
+    gpu_obj._call("psg_set_weights", gpu_obj._ctx, theta, 1e-3)
+    w = 1.0 / theta
+    ref = cpu_obj._scores(w, 1e-3)
+    got = gpu_obj._scores(w, 1e-3)
+    assert np.allclose(got, ref, rtol=1e-12, atol=0.0)
+    mask = np.zeros(4800, np.uint8)
+    mask[::3] = 1
+    d_full, d_core = np.empty(1600), np.empty(1600)
+    gpu_obj._call("psg_diag_terms", gpu_obj._ctx, mask, d_full, d_core)
+    A2 = lp.A.multiply(lp.A)
+    assert np.allclose(d_full, A2 @ w, rtol=1e-12)
+    assert np.allclose(d_core, A2 @ (w * mask), rtol=1e-12)
+    gpu_obj.close()
+
+
+def _dense_precond(A, core, w, delta, eta=0.0):
+    """Oracle P^-1 via LAPACK: D (D M D + eta I)^-1 D with the reference's M."""
+    Ac = A[:, core]
+    G = (Ac @ sp.diags(w[core]) @ Ac.T).toarray()
+    A2 = A.multiply(A)
+    d_full = A2 @ w
+    d_core = np.diag(G).copy()
+    M = G + np.diag(np.maximum(d_full - d_core, 0.0) + delta)
+    dsc = 1.0 / np.sqrt(np.diag(M))
+    Ms = dsc[:, None] * M * dsc[None, :] + eta * np.eye(M.shape[0])
+    cf = la.cho_factor(Ms, lower=True)
+    return lambda r: dsc * la.cho_solve(cf, dsc * r), np.linalg.cond(Ms)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("ordering,width,frac", [(1, 64, 1.0), (1, 16, 0.5), (0, 64, 0.7), (1, 1, 0.3)])
+def test_factor_apply_matches_lapack(gpu, precision, ordering, width, frac):
+    _say(f"factor/apply {precision} ordering={ordering} width={width} core={frac}")
+    m, n = 1200, 3600
+    _, lp = _lp(m, n, 5, 11)
+    A = lp.A
+    rng = np.random.default_rng(12)
+    cand = np.flatnonzero(np.diff(A.indptr) >= 2)
+    core = np.sort(rng.choice(cand, int(frac * m), replace=False))
+    theta = _theta(n, 13, spread=2.0)
+    delta = 1e-2
+    obj = HipSpectralStableColumnPreconditioner(A, protected=~lp.pos, precision=precision)
+    obj._call("psg_set_weights", obj._ctx, theta, delta)
+    plan = SymbolicPlan(A, core, ordering=ordering, max_width=width)
+    nnz = C.c_double()
+    status = obj._call("psg_factor", obj._ctx, plan.handle, 0.0, C.byref(nnz))
+    assert status == 0
+    assert int(nnz.value) == plan.info["nnz_L"]
+    apply_ref, cond = _dense_precond(A, core, 1.0 / theta, delta)
+    for trial in range(3):
+        r = rng.standard_normal(m)
+        got = obj._apply(r)
+        ref = apply_ref(r)
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        bound = 1e-9 if precision == "fp64" else 5e-7 * cond
+        assert err <= bound, (err, cond)
+    plan.close()
+    obj.close()
+
+
+def test_factor_reports_non_positive_pivot(gpu):
+    _say("factor non-PD (negative delta)")
+    _, lp = _lp(400, 1200, 5, 21)
+    obj = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos)
+    theta = np.full(1200, 1e6)
+    obj._call("psg_set_weights", obj._ctx, theta, -1.0)  # M = A W A' - I is indefinite
+    core = np.flatnonzero(np.diff(lp.A.indptr) >= 2)[:200]
+    plan = SymbolicPlan(lp.A, core)
+    status = obj._call("psg_factor", obj._ctx, plan.handle, 0.0, None)
+    assert status > 0
+    plan.close()
+    obj.close()
+
+
+def _pair(A, pos, **kw):
+    return (HipSpectralStableColumnPreconditioner(A, protected=~pos, **kw),
+            SpectralStableColumnPreconditionerCPU(A, protected=~pos, **kw))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_matches_oracle(gpu, precision):
+    _say(f"solve vs oracle {precision}")
+    m, n = 1600, 4800
+    _, lp = _lp(m, n, 5, 31)
+    g, c = _pair(lp.A, lp.pos, cg_tol=1e-9, cg_tol_max=1e-9)
+    g.close()
+    g = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, precision=precision,
+                                              cg_tol=1e-9, cg_tol_max=1e-9)
+    rng = np.random.default_rng(32)
+    theta = _theta(n, 33, spread=3.0)
+    for obj in (g, c):
+        obj.factorize(theta, 1e-6)
+    assert np.array_equal(g.core, c.core)
+    r1, r2 = rng.standard_normal(n), rng.standard_normal(m)
+    dxg, dyg = g.solve(r1, r2)
+    dxc, dyc = c.solve(r1, r2)
+    H = lambda v: lp.A @ ((1 / theta) * (lp.A.T @ v)) + 1e-6 * v
+    rhs = r2 - lp.A @ (r1 / theta)
+    assert np.linalg.norm(rhs - H(dyg)) <= 2e-9 * max(1.0, np.linalg.norm(rhs))
+    assert np.linalg.norm(dyg - dyc) <= 1e-6 * np.linalg.norm(dyc)
+    assert np.allclose(dxg, r1 / theta + (lp.A.T @ dyg) / theta, rtol=1e-10, atol=1e-12 * np.abs(dxg).max())
+    if precision == "fp64":
+        assert abs(g.cg_iterations[-1] - c.cg_iterations[-1]) <= 2
+    g.close()
+
+
+def test_pcg_info_codes_on_bad_rhs(gpu):
+    _say("pcg nan rhs")
+    _, lp = _lp(600, 1800, 5, 41)
+    g = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos)
+    g.factorize(np.ones(1800), 1.0)
+    r2 = np.zeros(600)
+    r2[3] = np.nan
+    with pytest.raises(P.ipm.KKTSolveError):
+        g.solve(np.zeros(1800), r2)
+    g.close()
+
+
+@pytest.mark.parametrize("m,n,precision", [(300, 900, "fp64"), (1600, 4800, "fp64"), (1600, 4800, "mixed")])
+def test_end_to_end_matches_highs(gpu, m, n, precision):
+    _say(f"end-to-end m={m} n={n} {precision}")
+    from scipy.optimize import linprog
+    raw, lp = _lp(m, n, 5, 0)
+    tol = P.lp.scaled_tolerance(lp, 1e-8)
+    solver = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, precision=precision,
+                                                   cg_tol=1e-8, cg_tol_max=1e-8)
+    sol = P.ipm.psipm_solve(lp, solver, tol=tol, compl="gap", maxit=160)
+    assert sol.status == "optimal"
+    info = P.lp.recover(lp, raw, sol.x, sol.y, sol.z, sol.w)
+    ref = linprog(raw.c, A_eq=raw.A, b_eq=raw.rl, bounds=(0, None), method="highs-ds")
+    assert abs(info["obj"] - ref.fun) <= 1e-8 * abs(ref.fun)
+    assert info["pinf"] <= 1e-8 and info["dinf"] <= 1e-8
+    solver.close()
