@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark of the PS-IPM hot path on one MI355X.
+
+Workload (one LP that fits one GPU): the random standard-form LP with
+m = 200,000 rows, n = 1,000,000 columns, 8 nonzeros per column (values
+N(0,1)), x0 ~ U[0.1,1], y0 ~ N(0,1), z0 ~ U[0.1,1], canonicalised and
+equilibrated exactly as the solver pipeline does.  The Newton matrix uses a
+synthetic IPM-like weight vector theta = 10^U(-4,4) (seeded), delta = 1e-6.
+The preconditioner core is the reference's score-ranked active-column set
+capped by a fill budget (nnz(L) <= --fill-budget).
+
+A step is one PCG iteration of the plain FP64 GPU path: H p (two gathers),
+the fused step and dot kernels, P^-1 r (both supernodal triangular solves),
+and the direction update, with every input resident in HBM.  `value` is
+PCG iterations per second.  The line also reports the mixed-precision
+rate, the factorisation time, the HBM roofline of the H v kernels (the
+hot path's HBM-bound kernel pair) and a CPU baseline: this project's FP64
+CPU restatement (scipy SpMV + SuperLU factor of the same active-column
+matrix, single thread) timed on a bounded sample of PCG iterations.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(m, n, k, seed):
+    import mf_psipm_amd as P
+
+    raw, *_ = P.generators.random_lp(m, n, k, seed)
+    lp = P.lp.to_standard_form(raw)
+    P.lp.scale_problem(lp)
+    return lp
+
+
+def hv_bytes(m, n, nnz, value_bytes):
+    """Algorithmic HBM bytes of one H v: both gathers read A once each
+    (value + int32 index per nonzero), the pointer arrays, and the vectors
+    (v read twice, W, t written and read, y written; FP64)."""
+    return 2 * nnz * (value_bytes + 4) + 4 * (m + n + 2) + 8 * (3 * m + 3 * n)
+
+
+def gpu_leg(lp, args, precision):
+    import ctypes as C
+
+    import numpy as np
+
+    from mf_psipm_amd import _lib
+    from mf_psipm_amd.preconditioner import HipSpectralStableColumnPreconditioner
+
+    m, n = lp.A.shape
+    rng = np.random.default_rng(args.seed + 1)
+    theta = 10.0 ** rng.uniform(-4.0, 4.0, n)
+    solver = HipSpectralStableColumnPreconditioner(
+        lp.A, protected=~lp.pos, precision=precision, structural_backbone=False,
+        core_factor=args.core_factor, max_factor_entries=args.fill_budget,
+        cg_tol=1e-12, cg_tol_max=1e-12, cg_maxit=args.steps, residual_replacement=0)
+    t0 = time.perf_counter()
+    solver.factorize(theta, 1e-6)
+    factor_wall = time.perf_counter() - t0
+    r1 = np.zeros(n)
+    r2 = rng.standard_normal(m)
+    solver._call("psg_solve_begin", solver._ctx, r1, r2)
+    # warmup
+    solver.cg_maxit = max(1, args.warmup)
+    solver._pcg(1e-30)
+    solver.cg_maxit = args.steps
+    res = _lib.PsgPcgResult()
+    t0 = time.perf_counter()
+    solver._call("psg_pcg", solver._ctx, 1e-30, args.steps, 0, 0.1, C.byref(res), None, None, None)
+    wall = time.perf_counter() - t0
+    dev = C.c_double()
+    solver._call("psg_last_timings", solver._ctx, C.byref(dev), None)
+    iters = max(1, res.iterations)
+    kt = np.zeros(4)
+    solver._call("psg_time_kernels", solver._ctx, 20, kt)
+    out = dict(
+        precision=precision, iterations=int(res.iterations), info=int(res.info), wall_s=wall,
+        device_ms=dev.value, it_per_s=iters / wall, factor_s=factor_wall,
+        factor_device_ms=solver.device_factor_ms, core=int(solver.core.size),
+        nnz_L=int(solver.nnz_factor), plan=dict(solver._plan.info), hv_ms=kt[0], hv_loop_ms=kt[1],
+        apply_ms=kt[2], dot_ms=kt[3], theta=theta, r2=r2, core_cols=solver.core.copy(),
+    )
+    solver.close()
+    return out
+
+
+def cpu_leg(lp, core, theta, r2, seconds):
+    """Oracle PCG iterations on the same matrix/core/weights (single thread)."""
+    import numpy as np
+
+    from oracle.ssp_cpu import SpectralStableColumnPreconditionerCPU
+
+    m, n = lp.A.shape
+    cpu = SpectralStableColumnPreconditionerCPU(lp.A, protected=~lp.pos, structural_backbone=False,
+                                                cg_tol=1e-12, cg_tol_max=1e-12)
+    cpu._weights = 1.0 / theta
+    cpu.core = np.asarray(core)
+    cpu._initialized = True
+    t0 = time.perf_counter()
+    cpu._factor_numeric(theta, 1e-6)
+    factor_s = time.perf_counter() - t0
+    rhs = r2.copy()
+    scale = max(1.0, float(np.linalg.norm(rhs)))
+    x = cpu.apply(rhs)
+    r = rhs - cpu.full_matvec(x)
+    z = cpu.apply(r)
+    p = z.copy()
+    rz = float(r @ z)
+    its = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        q = cpu.full_matvec(p)
+        alpha = rz / float(p @ q)
+        x += alpha * p
+        r -= alpha * q
+        _ = float(np.linalg.norm(r)) / scale
+        z = cpu.apply(r)
+        rz_new = float(r @ z)
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+        its += 1
+    dt = time.perf_counter() - t0
+    return dict(it_per_s=its / dt, iterations=its, seconds=dt, factor_s=factor_s)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--m", type=int, default=200_000)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--nnz-per-col", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--core-factor", type=float, default=0.1)
+    ap.add_argument("--fill-budget", type=float, default=2.0e8)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    lp = build_workload(args.m, args.n, args.nnz_per_col, args.seed)
+    m, n = lp.A.shape
+    nnz = lp.A.nnz
+    log(f"workload m={m} n={n} nnz={nnz} built in {time.perf_counter() - t0:.1f}s")
+    g64 = gpu_leg(lp, args, "fp64")
+    log(f"fp64: {g64['iterations']} it in {g64['wall_s']:.3f}s -> {g64['it_per_s']:.1f} it/s; "
+        f"core {g64['core']} nnzL {g64['nnz_L']:.3e} factor {g64['factor_s']:.2f}s; "
+        f"Hv {g64['hv_ms']:.4f} ms apply {g64['apply_ms']:.4f} ms")
+    mixed = None
+    if not args.no_mixed:
+        mixed = gpu_leg(lp, args, "mixed")
+        log(f"mixed: {mixed['it_per_s']:.1f} it/s; Hv(loop) {mixed['hv_loop_ms']:.4f} ms "
+            f"apply {mixed['apply_ms']:.4f} ms")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_leg(lp, g64["core_cols"], g64["theta"], g64["r2"], args.cpu_seconds)
+        log(f"cpu: {cpu['it_per_s']:.3f} it/s over {cpu['iterations']} it (factor {cpu['factor_s']:.1f}s)")
+
+    bytes_hv = hv_bytes(m, n, nnz, 8)
+    achieved = bytes_hv / (g64["hv_ms"] * 1e-3) / 1e9
+    line = {
+        "metric": "PCG iterations/s (plain FP64 GPU path), 1 MI355X",
+        "value": round(g64["it_per_s"], 3),
+        "unit": "PCG it/s",
+        "n_gpus": 1,
+        "steps": g64["iterations"],
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * g64["wall_s"] / max(1, g64["iterations"]), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"random LP m={m} n={n} nnz/col={args.nnz_per_col} seed={args.seed}",
+                   "theta": "10^U(-4,4) seeded, delta 1e-6", "core": g64["core"],
+                   "nnz_L": g64["nnz_L"], "fill_budget": args.fill_budget,
+                   "supernodes": g64["plan"]["nsuper"], "levels": g64["plan"]["nlevels"]},
+        "factor_s": round(g64["factor_s"], 4),
+        "mixed_it_per_s": None if mixed is None else round(mixed["it_per_s"], 3),
+        "kernel_ms": {"Hv": g64["hv_ms"], "apply_P": g64["apply_ms"], "dot": g64["dot_ms"],
+                      "Hv_mixed": None if mixed is None else mixed["hv_loop_ms"],
+                      "apply_P_mixed": None if mixed is None else mixed["apply_ms"]},
+        "roofline": {"bound": "hbm", "kernel": "H v (CSC gather + CSR gather)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": None if cpu is None else {
+            "value": round(cpu["it_per_s"], 4), "unit": "PCG it/s", "cores": 1, "kind": "port",
+            "sample": f"{cpu['iterations']} PCG iterations (~{args.cpu_seconds:.0f}s) of the FP64 CPU "
+                      f"oracle on the same matrix, core and weights; SuperLU factor excluded "
+                      f"({cpu['factor_s']:.1f}s)"},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -132,6 +132,14 @@ int psg_solve_finish(psg_ctx* ctx, double* dx, double* dy);
 int psg_pcg_residual(psg_ctx* ctx, double* residual);
 
 /* ---- timing ---------------------------------------------------------------- */
+/* Device time of the hot-path pieces, measured with HIP events on the
+ * library's own stream: out[0] = H v (both gathers, FP64 values),
+ * out[1] = H v with the PCG-loop value precision, out[2] = P^-1 r (both
+ * triangular solves incl. scaling/permutation), out[3] = one PCG step's
+ * vector kernels (axpy + dot + direction).  Each is the mean over `reps`
+ * back-to-back launches (ms).  No counterpart in the reference (its timers
+ * are perf_counter totals, PS:648-652); used by the benchmark. */
+int psg_time_kernels(psg_ctx* ctx, int32_t reps, double* out);
 /* device time (ms, HIP events) of the last psg_pcg and last psg_factor calls */
 int psg_last_timings(psg_ctx* ctx, double* pcg_ms, double* factor_ms);
 

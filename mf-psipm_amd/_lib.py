@@ -85,6 +85,7 @@ SIGNATURES = [
                           C.POINTER(PsgPcgResult), _opt(_f64p), _opt(_f64p), _opt(_f64p)]),
     ("psg_solve_finish", C.c_int, [C.c_void_p, _f64p, _f64p]),
     ("psg_pcg_residual", C.c_int, [C.c_void_p, _f64p]),
+    ("psg_time_kernels", C.c_int, [C.c_void_p, C.c_int32, _f64p]),
     ("psg_last_timings", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 ]
 

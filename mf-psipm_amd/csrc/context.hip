@@ -676,6 +676,31 @@ int psg_pcg_residual(psg_ctx* c, double* residual) {
   });
 }
 
+int psg_time_kernels(psg_ctx* c, int32_t reps, double* out) {
+  return guarded([&]() -> int {
+    if (!c || !out || reps < 1) throw Error(PSG_ERR_ARGUMENT, "psg_time_kernels: invalid arguments");
+    need_weights(c);
+    hipStream_t s = c->stream;
+    std::vector<double> ones(c->m, 1.0);
+    c->vin.upload(ones, s);
+    auto timed = [&](auto&& body) {
+      body();  // warm
+      HIP_CHECK(hipEventRecord(c->ev0, s));
+      for (int k = 0; k < reps; ++k) body();
+      HIP_CHECK(hipEventRecord(c->ev1, s));
+      HIP_CHECK(hipEventSynchronize(c->ev1));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      return (double)ms / reps;
+    };
+    out[0] = timed([&] { device_Hv(c, c->vin.p, c->vout.p, c->partA.p, false); });
+    out[1] = timed([&] { device_Hv(c, c->vin.p, c->vout.p, c->partA.p, true); });
+    out[2] = factor_ready(c) ? timed([&] { factor_apply(c, c->vin.p, c->vout.p); }) : -1.0;
+    out[3] = timed([&] { device_dot(c, c->vin.p, c->vout.p, c->m, c->partD.p); });
+    return PSG_OK;
+  });
+}
+
 int psg_last_timings(psg_ctx* c, double* pcg_ms, double* factor_ms) {
   return guarded([&]() -> int {
     if (!c) throw Error(PSG_ERR_ARGUMENT, "psg_last_timings: null context");

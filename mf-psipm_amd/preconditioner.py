@@ -387,15 +387,43 @@ class HipSpectralStableColumnPreconditioner:
 
     # ------------------------------------------------------------ factorisation
     def _ensure_plan(self):
+        """Symbolic plan of the current core, rebuilt only when the core changes.
+
+        With ``max_factor_entries`` set (no counterpart in the reference, whose
+        CHOLMOD factor of an m-column core is Theta(m^2) on random LPs), a core
+        whose predicted nnz(L) exceeds the budget loses its lowest-scored
+        non-mandatory columns (15% per round) until the plan fits, and the
+        core targets are capped at the size that fitted.
+        """
         key = self.core.tobytes()
         if self._plan is not None and key == self._plan_key:
             return self._plan
         if self._plan is not None:
             self._plan.close()
-        self._plan = SymbolicPlan(self.A, self.core, ordering=self.ordering, max_width=self.max_width,
-                                  dense_factor=self.dense_factor, lib=self._lib)
-        self._plan_key = key
-        self.n_symbolic += 1
+            self._plan = None
+        while True:
+            plan = SymbolicPlan(self.A, self.core, ordering=self.ordering, max_width=self.max_width,
+                                dense_factor=self.dense_factor, lib=self._lib)
+            self.n_symbolic += 1
+            budget = self.max_factor_entries
+            if budget is None or plan.info["nnz_L"] <= budget or self.core.size <= 1:
+                break
+            plan.close()
+            scores = getattr(self, "_last_scores", None)
+            local = np.searchsorted(self.candidates, self.core)
+            keep = max(1, int(0.85 * self.core.size))
+            optional = ~np.isin(self.core, self.mandatory)
+            rank = np.full(self.core.size, np.inf)
+            if scores is not None and scores.size == self.candidates.size:
+                rank[optional] = scores[local[optional]]
+            else:
+                rank[optional] = 0.0
+            order = np.lexsort((self.core, -rank))
+            self.core = np.sort(self.core[order[:keep]])
+            self.core_target = min(self.core_target, self.core.size)
+            self.max_core_target = min(self.max_core_target, self.core.size)
+        self._plan = plan
+        self._plan_key = self.core.tobytes()
         return self._plan
 
     def _factor_numeric(self, theta, delta):
