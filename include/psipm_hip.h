@@ -140,6 +140,12 @@ int psg_pcg_residual(psg_ctx* ctx, double* residual);
  * back-to-back launches (ms).  No counterpart in the reference (its timers
  * are perf_counter totals, PS:648-652); used by the benchmark. */
 int psg_time_kernels(psg_ctx* ctx, int32_t reps, double* out);
+/* Diagnostics: per-unit wall-clock ticks (100 MHz) of the last single-launch
+ * triangular solve, 6 per unit (forward start/ready/end, backward
+ * start/ready/end), recorded only when the factor was built with the
+ * environment variable PSG_SF_TRACE set; *nunits = 0 otherwise.  No
+ * counterpart in the reference. */
+int psg_solve_trace(psg_ctx* ctx, int64_t* out, int64_t cap, int64_t* nunits);
 /* device time (ms, HIP events) of the last psg_pcg and last psg_factor calls */
 int psg_last_timings(psg_ctx* ctx, double* pcg_ms, double* factor_ms);
 

@@ -86,6 +86,7 @@ SIGNATURES = [
     ("psg_solve_finish", C.c_int, [C.c_void_p, _f64p, _f64p]),
     ("psg_pcg_residual", C.c_int, [C.c_void_p, _f64p]),
     ("psg_time_kernels", C.c_int, [C.c_void_p, C.c_int32, _f64p]),
+    ("psg_solve_trace", C.c_int, [C.c_void_p, _i64p, C.c_int64, C.POINTER(C.c_int64)]),
     ("psg_last_timings", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 ]
 

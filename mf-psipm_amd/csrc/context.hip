@@ -507,6 +507,7 @@ int psg_apply_P(psg_ctx* c, const double* r, double* z) {
     factor_apply(c, c->vin.p, c->vout.p);
     c->vout.download(z, c->m, c->stream);
     HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (factor_solve_error(c)) throw Error(PSG_ERR_INTERNAL, "triangular solve dependency wait timed out");
     return PSG_OK;
   });
 }
@@ -637,6 +638,7 @@ int psg_pcg(psg_ctx* c, double tol, int32_t maxit, int32_t replacement, double g
     float ms = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_pcg_ms = ms;
+    if (factor_solve_error(c)) throw Error(PSG_ERR_INTERNAL, "triangular solve dependency wait timed out");
     if (history_out) std::copy(hist.begin(), hist.end(), history_out);
     res->info = info;
     res->iterations = it;

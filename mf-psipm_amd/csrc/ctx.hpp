@@ -61,6 +61,8 @@ int factor_numeric(psg_ctx* ctx, const Plan& plan, double eta, double* nnz_facto
 bool factor_ready(const psg_ctx* ctx);
 // z = D P^T L^-T L^-1 P D r on device vectors (m), on ctx->stream
 void factor_apply(psg_ctx* ctx, const double* r, double* z);
+// nonzero if a dependency wait of the last single-launch solve timed out
+int factor_solve_error(psg_ctx* ctx);
 // context.hip helpers used by factor.hip
 void diag_terms_device(psg_ctx* ctx, const uint8_t* mask_dev, double* full, double* core);
 }  // namespace psg

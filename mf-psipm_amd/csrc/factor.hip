@@ -24,6 +24,7 @@
 // (PSG_MIXED); the assembly sums and the scaling are always FP64.
 #include <algorithm>
 #include <climits>
+#include <numeric>
 #include <cmath>
 
 #include "ctx.hpp"
@@ -60,7 +61,16 @@ struct FactorState {
   int64_t npanel = 0, ndinv = 0;
   double nnz_L = 0.0;
   bool ready = false;
-  std::vector<int> single_node_level;  // 1 if the level holds exactly one supernode
+  std::vector<int> single_node_level;
+  // single-launch solve: unit lists, dependency counts, counters
+  int sf_units = 0;
+  DevBuf<int> su_s, su_r0, su_n, su_range, need, flags, bwd_order, iso_col, iso_doff;
+  int n_iso = 0;
+  DevBuf<long long> trace;  // allocated when PSG_SF_TRACE is set
+  DevBuf<double> sf_part64;
+  DevBuf<float> sf_part32;
+  int solve_mode = 1;  // 1 single-launch, 0 level-scheduled
+  int sf_grid = 1024, sf_sleep = 16, sf_batch = 1;  // single-launch launch shape  // 1 if the level holds exactly one supernode
 };
 
 // ---------------------------------------------------------------------------
@@ -436,6 +446,302 @@ k_bwd_finish(const int* __restrict__ nodes, const int* __restrict__ sunit_range,
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// single-launch (dependency-driven) triangular solves
+// ---------------------------------------------------------------------------
+// Work units (s, row block) are numbered children-first; workgroups take
+// units in ticket order, so a unit only ever waits for units with smaller
+// tickets, which are already running: progress is guaranteed without grid
+// barriers.  Dependencies are acquire/release counters at agent scope (the
+// scope that makes the per-XCD L2 caches agree); every wait is bounded and
+// raises *err instead of hanging.
+constexpr long kSpinLimit = 1L << 21;
+// Tickets may be taken a batch at a time: a workgroup processes its batch in
+// ticket order, so the smallest unfinished ticket is always the one its
+// holder is working on -- still deadlock-free.
+
+template <class T>
+struct SolveArgs {
+  int nunits;
+  const int* su_s;      // unit -> supernode
+  const int* su_r0;     // first below row of the unit
+  const int* su_n;      // rows of the unit
+  const int* su_range;  // [2s], [2s+1]: units of supernode s
+  const int* need;      // forward: number of (unit, target) contributions into s
+  const int* bwd_order; // backward ticket -> unit
+  const int* sfirst;
+  const int64_t* rptr;
+  const int* rows;
+  const int64_t* poff;
+  const int64_t* doff;
+  const int* snode_of;
+  const T* panels;
+  const T* dinv;
+  T* b;      // forward right-hand side (permuted, updated in place)
+  T* y;      // forward result
+  T* x;      // backward result
+  T* part;   // backward partials, kW per unit
+  int* cnt;  // forward arrivals per supernode
+  int* ydone;
+  int* pcnt; // backward arrivals per supernode
+  int* xdone;
+  int* ticket;  // [0] forward, [1] backward
+  int* err;
+  T* acc;       // backward: per-column sums of L_b' x (length m, zeroed)
+  int sleep_max;  // poll backoff cap (s_sleep units of 64 clocks)
+  int batch;      // tickets taken per atomic
+  long long* trace;  // optional: (start, ready, end) wall-clock ticks per unit and direction
+};
+
+// Poll with relaxed loads.  Ordering without cache-maintenance fences: every
+// datum that crosses workgroups moves through agent-scope atomics (coherent
+// per location), a producer's waves drain their memory counters
+// (s_waitcnt 0) before the barrier that precedes the flag update, and a
+// consumer issues its data loads only after the flag load has returned.
+__device__ inline bool wait_at_least(int* addr, int target, int* err, int sleep_max) {
+  int nap = 1;
+  for (long it = 0; it < kSpinLimit; ++it) {
+    if (__hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
+    // once any wait has timed out every other wait gives up at once
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    // exponential backoff keeps thousands of waiting workgroups from
+    // flooding the memory system that the critical path is using
+    for (int k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(2);
+    if (nap < sleep_max) nap <<= 1;
+  }
+  __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return false;
+}
+
+// Data that crosses workgroups moves through agent-scope atomic loads and
+// stores, which are coherent per location across the XCDs' L2 caches.
+template <class T>
+__device__ inline T load_agent(const T* p) {
+  return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ inline void store_agent(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// stage the w x w block inverse into the full 64 x 64 LDS tile (zero
+// padding outside w x w); 16 independent loads per lane
+template <class T>
+__device__ inline void stage_inverse(T (*sD)[kW + 1], const T* D, int w) {
+  T v[kW * kW / kBlock];
+#pragma unroll
+  for (int k = 0; k < kW * kW / kBlock; ++k) {
+    const int idx = threadIdx.x + k * kBlock;
+    const int r = idx % kW, c = idx / kW;
+    v[k] = (r < w && c < w) ? D[c * w + r] : T(0);
+  }
+#pragma unroll
+  for (int k = 0; k < kW * kW / kBlock; ++k) {
+    const int idx = threadIdx.x + k * kBlock;
+    sD[idx % kW][idx / kW] = v[k];
+  }
+}
+
+// sum_c P[c * nr] ys[c] over c < w with every load issued up front
+template <class T>
+__device__ inline T row_dot(const T* __restrict__ P, int64_t nr, const T* ys, int w) {
+  T v[kW];
+#pragma unroll
+  for (int c = 0; c < kW; ++c) v[c] = (c < w) ? P[(int64_t)c * nr] : T(0);
+  T acc0 = T(0), acc1 = T(0), acc2 = T(0), acc3 = T(0);
+#pragma unroll
+  for (int c = 0; c < kW; c += 4) {
+    acc0 += v[c] * ys[c];
+    acc1 += v[c + 1] * ys[c + 1];
+    acc2 += v[c + 2] * ys[c + 2];
+    acc3 += v[c + 3] * ys[c + 3];
+  }
+  return (acc0 + acc1) + (acc2 + acc3);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_fwd_sf(SolveArgs<T> a) {
+  __shared__ T sD[kW][kW + 1];
+  __shared__ T bs[kW], ys[kW];
+  __shared__ T ypart[kBlock / kWave][kW];
+  __shared__ int tgt[kBlock];
+  __shared__ int sh_u;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1), wv = tid / kWave;
+  int batch_next = 0, batch_end = 0;  // tickets are taken a batch at a time
+  for (;;) {
+    if (tid == 0) {
+      if (batch_next == batch_end) {
+        batch_next = atomicAdd(&a.ticket[0], a.batch);
+        batch_end = batch_next + a.batch;
+      }
+      sh_u = batch_next++;
+    }
+    __syncthreads();
+    const int u = sh_u;
+    if (u >= a.nunits) break;
+    if (a.trace && tid == 0) a.trace[6 * (int64_t)u] = wall_clock64();
+    const int s = a.su_s[u];
+    const int f = a.sfirst[s];
+    const int w = a.sfirst[s + 1] - f;
+    const int64_t nr = a.rptr[s + 1] - a.rptr[s];
+    const bool first = (u == a.su_range[2 * s]);
+    const int nrow = a.su_n[u];
+    // everything that does not depend on b_s is fetched before the wait
+    const int64_t r = w + a.su_r0[u] + tid;
+    T v[kW];
+    int row = -1, t = -1;
+    if (tid < nrow) {
+      const T* P = a.panels + a.poff[s] + r;
+#pragma unroll
+      for (int c = 0; c < kW; ++c) v[c] = (c < w) ? P[(int64_t)c * nr] : T(0);
+      row = a.rows[a.rptr[s] + r];
+      t = a.snode_of[row];
+    }
+    stage_inverse(sD, a.dinv + a.doff[s], w);
+    if (tid == 0) {
+      wait_at_least(&a.cnt[s], a.need[s], a.err, a.sleep_max);
+      if (a.trace) a.trace[6 * (int64_t)u + 1] = wall_clock64();
+    }
+    __syncthreads();
+    if (tid < kW) bs[tid] = (tid < w) ? load_agent(a.b + f + tid) : T(0);
+    __syncthreads();
+    // ys = L_ss^-1 b_s: wave wv sums columns [16 wv, 16 wv + 16) for row `lane`
+    {
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < kW / 4; ++k) {
+        const int c = wv * (kW / 4) + k;
+        acc += sD[lane][c] * bs[c];
+      }
+      ypart[wv][lane] = acc;
+    }
+    __syncthreads();
+    if (tid < kW) {
+      const T y = (tid < w) ? (ypart[0][tid] + ypart[1][tid]) + (ypart[2][tid] + ypart[3][tid]) : T(0);
+      ys[tid] = y;
+      if (first && tid < w) a.y[f + tid] = y;  // read by the backward launch
+    }
+    __syncthreads();
+    if (tid < nrow) {
+      T acc0 = T(0), acc1 = T(0), acc2 = T(0), acc3 = T(0);
+#pragma unroll
+      for (int c = 0; c < kW; c += 4) {
+        acc0 += v[c] * ys[c];
+        acc1 += v[c + 1] * ys[c + 1];
+        acc2 += v[c + 2] * ys[c + 2];
+        acc3 += v[c + 3] * ys[c + 3];
+      }
+      __hip_atomic_fetch_add(a.b + row, -((acc0 + acc1) + (acc2 + acc3)), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    }
+    tgt[tid] = t;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's atomics performed before the barrier
+    __syncthreads();
+    if (tid < nrow && (tid == 0 || tgt[tid - 1] != t))
+      __hip_atomic_fetch_add(&a.cnt[t], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (a.trace && tid == 0) a.trace[6 * (int64_t)u + 2] = wall_clock64();
+    __syncthreads();
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
+  __shared__ T sD[kW][kW + 1];
+  __shared__ T xr[kBlock];
+  __shared__ T cs[kW];
+  __shared__ T part_sh[kBlock / kWave][kWave];
+  __shared__ int tg[kBlock];
+  __shared__ int sh_u, sh_last;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1), wv = tid / kWave;
+  int batch_next = 0, batch_end = 0;
+  for (;;) {
+    if (tid == 0) {
+      if (batch_next == batch_end) {
+        batch_next = atomicAdd(&a.ticket[1], a.batch);
+        batch_end = batch_next + a.batch;
+      }
+      sh_u = batch_next++;
+    }
+    __syncthreads();
+    const int ticket = sh_u;
+    if (ticket >= a.nunits) break;
+    const int u = a.bwd_order[ticket];  // parents first
+    if (a.trace && tid == 0) a.trace[6 * (int64_t)u + 3] = wall_clock64();
+    const int s = a.su_s[u];
+    const int f = a.sfirst[s];
+    const int w = a.sfirst[s + 1] - f;
+    const int64_t nr = a.rptr[s + 1] - a.rptr[s];
+    const int nrow = a.su_n[u];
+    const int64_t r0 = w + a.su_r0[u];
+    // wait for the ancestors owning this block's rows
+    int row = -1;
+    if (tid < nrow) {
+      row = a.rows[a.rptr[s] + r0 + tid];
+      tg[tid] = a.snode_of[row];
+    }
+    // panel segment of this lane's column, fetched before waiting on x
+    const int i0 = wv * kWave;
+    const int cnt_rows = max(0, min(kWave, nrow - i0));
+    T v[kWave];
+    {
+      const T* P = a.panels + a.poff[s] + (int64_t)lane * nr + r0 + i0;
+      const bool col_ok = lane < w && cnt_rows > 0;
+#pragma unroll
+      for (int i = 0; i < kWave; ++i) v[i] = (col_ok && i < cnt_rows) ? P[i] : T(0);
+    }
+    __syncthreads();
+    if (tid < nrow && (tid == 0 || tg[tid - 1] != tg[tid])) wait_at_least(&a.xdone[tg[tid]], 1, a.err, a.sleep_max);
+    __syncthreads();
+    if (a.trace && tid == 0) a.trace[6 * (int64_t)u + 4] = wall_clock64();
+    xr[tid] = (tid < nrow) ? load_agent(a.x + row) : T(0);
+    __syncthreads();
+    // partial[c] = sum_rows L[r][c] x[row]: lane = column c, wave wv = rows
+    // [64 wv, 64 wv + 64) -- each lane streams a contiguous column segment
+    {
+      T acc0 = T(0), acc1 = T(0);
+#pragma unroll
+      for (int i = 0; i < kWave; i += 2) {
+        acc0 += v[i] * xr[i0 + i];
+        acc1 += v[i + 1] * xr[i0 + i + 1];
+      }
+      part_sh[wv][lane] = acc0 + acc1;
+    }
+    __syncthreads();
+    if (tid < w && nrow > 0) {
+      const T sum = (part_sh[0][tid] + part_sh[1][tid]) + (part_sh[2][tid] + part_sh[3][tid]);
+      __hip_atomic_fetch_add(a.acc + f + tid, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tid == 0) {
+      const int total = a.su_range[2 * s + 1] - a.su_range[2 * s];
+      const int old = __hip_atomic_fetch_add(&a.pcnt[s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      sh_last = (old == total - 1);
+    }
+    __syncthreads();
+    if (sh_last) {
+      stage_inverse(sD, a.dinv + a.doff[s], w);
+      if (tid < w) {
+        cs[tid] = a.y[f + tid] - load_agent(a.acc + f + tid);
+      }
+      __syncthreads();
+      if (tid < w) {
+        T acc = T(0);
+        for (int j = tid; j < w; ++j) acc += sD[j][tid] * cs[j];
+        store_agent(a.x + f + tid, acc);
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (tid == 0) __hip_atomic_store(&a.xdone[s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (a.trace && tid == 0) a.trace[6 * (int64_t)u + 5] = wall_clock64();
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
@@ -502,6 +808,102 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
       sunit_range[2 * sn + 1] = sunit_first[sn] + count[sn];
     }
   }
+  // single-launch solve units: first below block 64 rows (it feeds the parent,
+  // the critical path), then 256-row blocks; a supernode without below rows
+  // still owns one (empty) unit for its diagonal solve
+  {
+    std::vector<int> us, ur0, un, range(2 * P.nsuper), need(P.nsuper, 0);
+    // forward tickets follow the tree level (height above the leaves), so a
+    // unit is taken roughly when it becomes ready and no ready work starves
+    // behind waiting units; the backward order is set below from depth
+    std::vector<int> level(P.nsuper, 0), depth(P.nsuper, 0), order(P.nsuper);
+    for (int sn = 0; sn < P.nsuper; ++sn)
+      if (P.sparent[sn] >= 0) level[P.sparent[sn]] = std::max(level[P.sparent[sn]], level[sn] + 1);
+    for (int sn = P.nsuper - 1; sn >= 0; --sn)
+      if (P.sparent[sn] >= 0) depth[sn] = depth[P.sparent[sn]] + 1;
+    // isolated 1x1 supernodes (rows no active column touches) are solved
+    // elementwise, x = b / l_kk^2, outside the dependency-driven kernels
+    std::vector<int> iso_col, iso_doff;
+    std::vector<char> has_child(P.nsuper, 0);
+    for (int sn = 0; sn < P.nsuper; ++sn)
+      if (P.sparent[sn] >= 0) has_child[P.sparent[sn]] = 1;
+    order.clear();
+    for (int sn = 0; sn < P.nsuper; ++sn) {
+      const bool iso = P.sfirst[sn + 1] - P.sfirst[sn] == 1 && P.rptr[sn + 1] - P.rptr[sn] == 1 &&
+                       !has_child[sn] && P.sparent[sn] < 0;
+      if (iso) {
+        iso_col.push_back(P.sfirst[sn]);
+        iso_doff.push_back((int)doff[sn]);
+      } else {
+        order.push_back(sn);
+      }
+    }
+    // ties: the longer remaining dependency chain first
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      return level[x] < level[y] || (level[x] == level[y] && depth[x] > depth[y]);
+    });
+    fs->iso_col.upload(iso_col, s);
+    fs->iso_doff.upload(iso_doff, s);
+    fs->n_iso = (int)iso_col.size();
+    for (int sn : order) {
+      const int w = P.sfirst[sn + 1] - P.sfirst[sn];
+      const int64_t nb = (P.rptr[sn + 1] - P.rptr[sn]) - w;
+      range[2 * sn] = (int)us.size();
+      int64_t r0 = 0;
+      do {
+        const int64_t blk = (r0 == 0) ? std::min<int64_t>(nb, kW) : std::min<int64_t>(nb - r0, kRB);
+        us.push_back(sn);
+        ur0.push_back((int)r0);
+        un.push_back((int)blk);
+        // distinct target supernodes of this block
+        int last = -1;
+        for (int64_t q = 0; q < blk; ++q) {
+          const int row = P.rows[P.rptr[sn] + w + r0 + q];
+          const int t = P.snode_of[row];
+          if (t != last) {
+            need[t]++;
+            last = t;
+          }
+        }
+        r0 += blk;
+      } while (r0 < nb);
+      range[2 * sn + 1] = (int)us.size();
+    }
+    fs->sf_units = (int)us.size();
+    // backward: parents first by depth below the root
+    std::vector<int> border(us.size());
+    std::iota(border.begin(), border.end(), 0);
+    std::stable_sort(border.begin(), border.end(), [&](int x, int y) {
+      const int a = us[x], b = us[y];
+      return depth[a] < depth[b] || (depth[a] == depth[b] && level[a] > level[b]);
+    });
+    fs->bwd_order.upload(border, s);
+    fs->su_s.upload(us, s);
+    fs->su_r0.upload(ur0, s);
+    fs->su_n.upload(un, s);
+    fs->su_range.upload(range, s);
+    fs->need.upload(need, s);
+    fs->flags.alloc(4 * (size_t)P.nsuper + 8);
+    if (c->precision == PSG_MIXED) {
+      fs->sf_part32.alloc((size_t)fs->sf_units * kW);
+      fs->sf_part64.release();
+    } else {
+      fs->sf_part64.alloc((size_t)fs->sf_units * kW);
+      fs->sf_part32.release();
+    }
+    const char* mode = getenv("PSG_SOLVE");
+    fs->solve_mode = (mode && mode[0] == 'l') ? 0 : 1;
+    auto knob = [](const char* name, int dflt, int lo, int hi) {
+      const char* v = getenv(name);
+      int x = v ? atoi(v) : dflt;
+      return std::max(lo, std::min(hi, x));
+    };
+    fs->sf_grid = knob("PSG_SF_GRID", 1024, 1, 65536);
+    fs->sf_sleep = knob("PSG_SF_SLEEP", 16, 1, 1024);
+    fs->sf_batch = knob("PSG_SF_BATCH", 1, 1, 64);
+    if (getenv("PSG_SF_TRACE")) fs->trace.alloc(6 * (size_t)fs->sf_units);
+    else fs->trace.release();
+  }
   fs->perm.upload(P.perm, s);
   fs->sfirst.upload(P.sfirst, s);
   fs->rows.upload(P.rows, s);
@@ -532,14 +934,16 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
   if (c->precision == PSG_MIXED) {
     fs->panels32.alloc(std::max<int64_t>(1, fs->npanel));
     fs->dinv32.alloc(std::max<int64_t>(1, fs->ndinv));
-    for (auto* b : {&fs->vb32, &fs->vy32, &fs->vx32}) b->alloc(P.m);
+    fs->vb32.alloc(2 * (size_t)P.m);
+    for (auto* b : {&fs->vy32, &fs->vx32}) b->alloc(P.m);
     fs->part32.alloc(npart);
     fs->panels64.release();
     fs->dinv64.release();
   } else {
     fs->panels64.alloc(std::max<int64_t>(1, fs->npanel));
     fs->dinv64.alloc(std::max<int64_t>(1, fs->ndinv));
-    for (auto* b : {&fs->vb64, &fs->vy64, &fs->vx64}) b->alloc(P.m);
+    fs->vb64.alloc(2 * (size_t)P.m);
+    for (auto* b : {&fs->vy64, &fs->vx64}) b->alloc(P.m);
     fs->part64.alloc(npart);
     fs->panels32.release();
     fs->dinv32.release();
@@ -645,6 +1049,100 @@ static void apply_run(psg_ctx* c, const double* r, double* z, const T* panels, c
   HIP_LAUNCH_CHECK();
 }
 
+
+// isolated 1x1 supernodes: x_k = dinv_k^2 b_k (forward and backward at once)
+template <class T>
+__global__ void k_iso_solve(int n, const int* __restrict__ col, const int* __restrict__ doff,
+                            const T* __restrict__ dinv, const T* __restrict__ b, T* __restrict__ y,
+                            T* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = col[i];
+    const T d = dinv[doff[i]];
+    const T yk = d * b[k];
+    y[k] = yk;
+    x[k] = d * yk;
+  }
+}
+
+template <class T>
+static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels, const T* dinv, T* vb, T* vy,
+                         T* vx, T* part) {
+  FactorState* fs = c->fs;
+  hipStream_t s = c->stream;
+  const int m = c->m;
+  const int ns = fs->nsuper;
+  HIP_CHECK(hipMemsetAsync(fs->flags.p, 0, fs->flags.n * sizeof(int), s));
+  hipLaunchKernelGGL((k_perm_in<T>), dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, fs->perm.p, c->dsc.p,
+                     r, vb);
+  HIP_LAUNCH_CHECK();
+  SolveArgs<T> a;
+  a.nunits = fs->sf_units;
+  a.su_s = fs->su_s.p;
+  a.su_r0 = fs->su_r0.p;
+  a.su_n = fs->su_n.p;
+  a.su_range = fs->su_range.p;
+  a.need = fs->need.p;
+  a.bwd_order = fs->bwd_order.p;
+  a.sfirst = fs->sfirst.p;
+  a.rptr = fs->rptr.p;
+  a.rows = fs->rows.p;
+  a.poff = fs->poff.p;
+  a.doff = fs->doff.p;
+  a.snode_of = fs->snode_of.p;
+  a.panels = panels;
+  a.dinv = dinv;
+  a.b = vb;
+  a.y = vy;
+  a.x = vx;
+  a.part = part;
+  a.cnt = fs->flags.p;
+  a.ydone = fs->flags.p + ns;
+  a.pcnt = fs->flags.p + 2 * ns;
+  a.xdone = fs->flags.p + 3 * ns;
+  a.ticket = fs->flags.p + 4 * ns;
+  a.err = fs->flags.p + 4 * ns + 4;
+  a.acc = vb + m;  // second half of the solve buffer, zeroed below
+  a.sleep_max = fs->sf_sleep;
+  a.batch = fs->sf_batch;
+  a.trace = fs->trace.p;
+  HIP_CHECK(hipMemsetAsync(a.acc, 0, (size_t)m * sizeof(T), s));
+  const int grid = std::max(1, std::min(fs->sf_units, fs->sf_grid));
+  if (fs->n_iso) {
+    hipLaunchKernelGGL((k_iso_solve<T>), dim3(grid_for(fs->n_iso, kBlock, 4096)), dim3(kBlock), 0, s, fs->n_iso,
+                       fs->iso_col.p, fs->iso_doff.p, dinv, vb, vy, vx);
+    HIP_LAUNCH_CHECK();
+  }
+  if (fs->sf_units) {
+    hipLaunchKernelGGL((k_fwd_sf<T>), dim3(grid), dim3(kBlock), 0, s, a);
+    HIP_LAUNCH_CHECK();
+  }
+  if (fs->sf_units) {
+    hipLaunchKernelGGL((k_bwd_sf<T>), dim3(grid), dim3(kBlock), 0, s, a);
+    HIP_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL((k_perm_out<T>), dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, fs->perm.p,
+                     c->dsc.p, vx, z);
+  HIP_LAUNCH_CHECK();
+}
+
+int factor_solve_error(psg_ctx* c) {
+  FactorState* fs = c->fs;
+  if (!fs || fs->solve_mode == 0 || !fs->flags.p) return 0;
+  int e = 0;
+  HIP_CHECK(hipMemcpyAsync(&e, fs->flags.p + 4 * fs->nsuper + 4, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return e;
+}
+
+int64_t factor_trace(psg_ctx* c, long long* out, int64_t cap) {
+  FactorState* fs = c->fs;
+  if (!fs || !fs->trace.p) return 0;
+  const int64_t n = std::min<int64_t>(cap, 6 * (int64_t)fs->sf_units);
+  HIP_CHECK(hipMemcpyAsync(out, fs->trace.p, n * sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return fs->sf_units;
+}
+
 void factor_destroy(psg_ctx* c) {
   delete c->fs;
   c->fs = nullptr;
@@ -675,6 +1173,13 @@ int factor_numeric(psg_ctx* c, const Plan& P, double eta, double* nnz_factor) {
 
 void factor_apply(psg_ctx* c, const double* r, double* z) {
   FactorState* fs = c->fs;
+  if (fs->solve_mode == 1) {
+    if (c->precision == PSG_MIXED)
+      apply_run_sf<float>(c, r, z, fs->panels32.p, fs->dinv32.p, fs->vb32.p, fs->vy32.p, fs->vx32.p, fs->sf_part32.p);
+    else
+      apply_run_sf<double>(c, r, z, fs->panels64.p, fs->dinv64.p, fs->vb64.p, fs->vy64.p, fs->vx64.p, fs->sf_part64.p);
+    return;
+  }
   if (c->precision == PSG_MIXED)
     apply_run<float>(c, r, z, fs->panels32.p, fs->dinv32.p, fs->vb32.p, fs->vy32.p, fs->vx32.p, fs->part32.p);
   else
@@ -684,6 +1189,14 @@ void factor_apply(psg_ctx* c, const double* r, double* z) {
 }  // namespace psg
 
 using namespace psg;
+
+extern "C" int psg_solve_trace(psg_ctx* c, int64_t* out, int64_t cap, int64_t* nunits) {
+  return guarded([&]() -> int {
+    if (!c || !out || !nunits) throw Error(PSG_ERR_ARGUMENT, "psg_solve_trace: null argument");
+    *nunits = factor_trace(c, reinterpret_cast<long long*>(out), cap);
+    return PSG_OK;
+  });
+}
 
 extern "C" int psg_factor(psg_ctx* c, const psg_plan* plan, double eta, double* nnz_factor) {
   return guarded([&]() -> int {

@@ -302,27 +302,114 @@ std::vector<int> etree_of_gram(int m, const std::vector<std::vector<int>>& cols_
   return parent;
 }
 
-std::vector<int> postorder(int m, const std::vector<int>& parent) {
-  std::vector<int> head(m, -1), next(m, -1), post;
-  post.reserve(m);
-  for (int j = m - 1; j >= 0; --j) {
-    if (parent[j] == -1) continue;
-    next[j] = head[parent[j]];
-    head[parent[j]] = j;
+// Relaxed amalgamation: a supernode whose columns end right before those of
+// its parent (its last child in postorder) is merged into the parent when the
+// explicit zeros this adds stay small for the merged width.  Fewer, wider
+// supernodes shorten the dependency chains of the numeric factor and of the
+// triangular solves; structurally zero entries stay exactly zero in floating
+// point, so the factor is unchanged.
+void relax_supernodes(std::vector<int>& sfirst, std::vector<std::vector<int>>& srows,
+                      const std::vector<int>& par, int wmax) {
+  const int ns = (int)sfirst.size() - 1;
+  std::vector<int> nf;                    // merged first columns
+  std::vector<std::vector<int>> nrows;    // merged structures
+  std::vector<double> truenz;             // true entries per merged supernode
+  for (int p = 0; p < ns; ++p) {
+    const int fp = sfirst[p];
+    const int wp = sfirst[p + 1] - fp;
+    const double rp = (double)srows[p].size();
+    double st_p = 0.0;
+    for (int k = 0; k < wp; ++k) st_p += rp - k;
+    bool merged = false;
+    if (!nf.empty()) {
+      const int q = (int)nf.size() - 1;
+      const int fq = nf[q];
+      const int lq = fp - 1;               // open supernode ends right before p
+      const int wq = lq - fq + 1;
+      const int wnew = wq + wp;
+      if (par[lq] >= fp && par[lq] < fp + wp && wnew <= wmax) {
+        double st_q = 0.0;
+        for (int k = 0; k < wq; ++k) st_q += (wq + rp) - k;
+        const double st = st_q + st_p;
+        const double tr = truenz[q] + st_p;
+        const double frac = st > 0 ? (st - tr) / st : 0.0;
+        if (wnew <= 4 || (wnew <= 16 && frac <= 0.8) || (wnew <= 48 && frac <= 0.1) || frac <= 0.05) {
+          std::vector<int> R;
+          R.reserve(wq + srows[p].size());
+          for (int c = fq; c <= lq; ++c) R.push_back(c);
+          R.insert(R.end(), srows[p].begin(), srows[p].end());
+          nrows[q].swap(R);
+          truenz[q] = tr;
+          merged = true;
+        }
+      }
+    }
+    if (!merged) {
+      nf.push_back(fp);
+      nrows.push_back(std::move(srows[p]));
+      truenz.push_back(st_p);
+    }
+    std::vector<int>().swap(srows[p]);
   }
-  std::vector<int> stack;
+  nf.push_back(sfirst[ns]);
+  sfirst.swap(nf);
+  srows.swap(nrows);
+}
+
+// Split supernodes wider than wmax into near-equal chunks; chunk i keeps
+// the suffix of the row structure that starts at its first column.
+void split_supernodes(std::vector<int>& sfirst, std::vector<std::vector<int>>& srows, int wmax) {
+  const int ns = (int)sfirst.size() - 1;
+  std::vector<int> nf;
+  std::vector<std::vector<int>> nrows;
+  for (int q = 0; q < ns; ++q) {
+    const int f = sfirst[q], w = sfirst[q + 1] - f;
+    const int pieces = (w + wmax - 1) / wmax;
+    int c0 = 0;
+    for (int i = 0; i < pieces; ++i) {
+      const int wi = w / pieces + (i < w % pieces ? 1 : 0);
+      nf.push_back(f + c0);
+      nrows.emplace_back(srows[q].begin() + c0, srows[q].end());
+      c0 += wi;
+    }
+    std::vector<int>().swap(srows[q]);
+  }
+  nf.push_back(sfirst[ns]);
+  sfirst.swap(nf);
+  srows.swap(nrows);
+}
+
+// Postorder of the elimination forest that visits the children of each node
+// by increasing subtree height, so the tallest child comes last: long chains
+// then occupy consecutive columns and can merge into wide supernodes.
+std::vector<int> postorder(int m, const std::vector<int>& parent) {
+  std::vector<int> height(m, 1);
+  for (int j = 0; j < m; ++j)  // parents follow children in the input numbering
+    if (parent[j] != -1) height[parent[j]] = std::max(height[parent[j]], height[j] + 1);
+  std::vector<int> cptr(m + 1, 0), kids(m);
+  for (int j = 0; j < m; ++j)
+    if (parent[j] != -1) cptr[parent[j] + 1]++;
+  for (int j = 0; j < m; ++j) cptr[j + 1] += cptr[j];
+  {
+    std::vector<int> fill(cptr.begin(), cptr.end() - 1);
+    for (int j = 0; j < m; ++j)
+      if (parent[j] != -1) kids[fill[parent[j]]++] = j;
+  }
+  for (int j = 0; j < m; ++j)
+    std::stable_sort(kids.begin() + cptr[j], kids.begin() + cptr[j + 1],
+                     [&](int x, int y) { return height[x] < height[y]; });
+  std::vector<int> post, next_kid(cptr.begin(), cptr.end() - 1), stack;
+  post.reserve(m);
   for (int r = 0; r < m; ++r) {
     if (parent[r] != -1) continue;
     stack.push_back(r);
     while (!stack.empty()) {
-      int p = stack.back();
-      int c = head[p];
-      if (c == -1) {
-        stack.pop_back();
-        post.push_back(p);
+      const int v = stack.back();
+      if (next_kid[v] < cptr[v + 1]) {
+        stack.push_back(kids[next_kid[v]++]);
       } else {
-        head[p] = next[c];
-        stack.push_back(c);
+        stack.pop_back();
+        post.push_back(v);
       }
     }
   }
@@ -402,6 +489,7 @@ Plan build_plan(int m, const int64_t* a_colptr, const int* a_rowind, const int* 
 
   // ---- 4. supernodes --------------------------------------------------------
   const int wmax = std::max(1, opt.max_width);
+  const int wfund = 1 << 30;  // fundamental supernodes are split to wmax only at the end
   std::vector<int> sfirst;
   std::vector<std::vector<int>> srows;  // full row structure (starts with own cols)
   std::vector<int> mark(m, -1);
@@ -413,7 +501,7 @@ Plan build_plan(int m, const int64_t* a_colptr, const int* a_rowind, const int* 
     // does k extend the current supernode?
     bool extend = false;
     if (cur >= 0 && k > 0 && par[k - 1] == k && nchild[k] == 1 &&
-        snode_of[k - 1] == cur && (k - sfirst[cur]) < wmax) {
+        snode_of[k - 1] == cur && (k - sfirst[cur]) < wfund) {
       // own(k) must be inside Struct(cur)
       const auto& R = srows[cur];
       extend = true;
@@ -449,8 +537,12 @@ Plan build_plan(int m, const int64_t* a_colptr, const int* a_rowind, const int* 
     srows.push_back(std::move(R));
   }
   if (cur >= 0 && par[m - 1] >= 0) child_super[par[m - 1]].push_back(cur);
-  const int ns = (int)sfirst.size();
   sfirst.push_back(m);
+  if (opt.relax) relax_supernodes(sfirst, srows, par, wfund);
+  split_supernodes(sfirst, srows, wmax);
+  const int ns = (int)sfirst.size() - 1;
+  for (int q = 0; q < ns; ++q)
+    for (int k = sfirst[q]; k < sfirst[q + 1]; ++k) snode_of[k] = q;
   // the first w rows of each structure must be the supernode's own columns;
   // extension guaranteed rows k..last are present (they are parents).
   P.nsuper = ns;

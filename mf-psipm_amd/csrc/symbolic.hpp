@@ -10,6 +10,7 @@ struct PlanOptions {
   int ordering = 1;          // 0 natural, 1 minimum degree
   int max_width = 64;        // supernode column cap (dense panel kernels use LDS)
   double dense_factor = 10;  // rows with degree > dense_factor*sqrt(m) are ordered last
+  bool relax = true;         // relaxed supernode amalgamation
 };
 
 // Supernodal plan of M = A_C W_C A_C^T + diag, in the permuted numbering

@@ -205,3 +205,34 @@ def test_end_to_end_matches_highs(gpu, m, n, precision):
     assert abs(info["obj"] - ref.fun) <= 1e-8 * abs(ref.fun)
     assert info["pinf"] <= 1e-8 and info["dinf"] <= 1e-8
     solver.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("m,n,k,frac", [(1500, 4500, 5, 1.0), (6000, 30000, 8, 0.25)])
+def test_single_launch_solve_matches_level_scheduled(gpu, monkeypatch, precision, m, n, k, frac):
+    _say(f"single-launch vs level solve {precision} m={m} core={frac}")
+    _, lp = _lp(m, n, k, 51)
+    A = lp.A
+    rng = np.random.default_rng(52)
+    cand = np.flatnonzero(np.diff(A.indptr) >= 2)
+    core = np.sort(rng.choice(cand, int(frac * m), replace=False))
+    theta = _theta(n, 53, spread=2.0)
+    outs = []
+    for mode in ("level", "single"):
+        monkeypatch.setenv("PSG_SOLVE", mode)
+        obj = HipSpectralStableColumnPreconditioner(A, protected=~lp.pos, precision=precision)
+        obj._call("psg_set_weights", obj._ctx, theta, 1e-2)
+        plan = SymbolicPlan(A, core)
+        assert obj._call("psg_factor", obj._ctx, plan.handle, 0.0, None) == 0
+        r = np.random.default_rng(54).standard_normal(m)
+        outs.append([obj._apply(r) for _ in range(2)])
+        plan.close()
+        obj.close()
+    (l1, l2), (s1, s2) = outs
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert np.linalg.norm(s1 - l1) <= tol * np.linalg.norm(l1)
+    assert np.linalg.norm(s2 - s1) <= tol * np.linalg.norm(s1)
+    if m <= 2000:
+        apply_ref, cond = _dense_precond(A, core, 1.0 / theta, 1e-2)
+        ref = apply_ref(r)
+        assert np.linalg.norm(s1 - ref) <= (1e-9 if precision == "fp64" else 5e-7 * cond) * np.linalg.norm(ref)
