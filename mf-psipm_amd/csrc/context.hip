@@ -27,74 +27,161 @@ namespace psg {
 
 // out_j = scale_j * (sum_p val[p] v[idx[p]] + add_j)   over segments j
 // (CSC: segments are columns -> W o (A^T v); with add = r1 it is dx of SSP:716)
-template <int G, class TV>
+// A group of G lanes owns a segment; each group handles U segments per pass
+// and issues every independent load of a stage before consuming any, so a
+// wave keeps U dependent chains (pointer -> index/value -> gather) in flight.
+template <int G, int U, class TV>
 __global__ void __launch_bounds__(kBlock)
 k_seg_gather(int nseg, const int* __restrict__ ptr, const int* __restrict__ idx,
              const TV* __restrict__ val, const double* __restrict__ v,
              const double* __restrict__ scale, const double* __restrict__ add,
              double* __restrict__ out) {
   const int lane = threadIdx.x % G;
-  const int groups_per_block = kBlock / G;
-  for (int64_t seg = (int64_t)blockIdx.x * groups_per_block + threadIdx.x / G; seg < nseg;
-       seg += (int64_t)gridDim.x * groups_per_block) {
-    const int b = ptr[seg], e = ptr[seg + 1];
-    double s = 0.0;
-    for (int p = b + lane; p < e; p += G) s += (double)val[p] * v[idx[p]];
+  const int gpb = kBlock / G;
+  const int g = threadIdx.x / G;
+  for (int64_t tile = (int64_t)blockIdx.x * U * gpb; tile < nseg; tile += (int64_t)gridDim.x * U * gpb) {
+    int b[U], e[U];
 #pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    for (int u = 0; u < U; ++u) {
+      const int64_t seg = tile + (int64_t)u * gpb + g;
+      b[u] = seg < nseg ? ptr[seg] : 0;
+      e[u] = seg < nseg ? ptr[seg + 1] : 0;
+    }
+    int ix[U];
+    double a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int p = b[u] + lane;
+      const bool ok = p < e[u];
+      ix[u] = ok ? idx[p] : 0;
+      a[u] = ok ? (double)val[p] : 0.0;
+    }
+    double sum[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) sum[u] = a[u] * v[ix[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      for (int p = b[u] + lane + G; p < e[u]; p += G) sum[u] += (double)val[p] * v[idx[p]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) sum[u] += __shfl_xor(sum[u], o, 64);
+    }
     if (lane == 0) {
-      double a = add ? add[seg] : 0.0;
-      out[seg] = (scale ? scale[seg] : 1.0) * (s + a);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t seg = tile + (int64_t)u * gpb + g;
+        if (seg < nseg) out[seg] = (scale ? scale[seg] : 1.0) * (sum[u] + (add ? add[seg] : 0.0));
+      }
     }
   }
 }
 
 enum CsrMode { kModeHv = 0, kModeResidual = 1, kModeRhs = 2 };
 
-// Row gather y_i = sum_p val[p] t[idx[p]] with a fused epilogue:
+// Row gather over one column block of A (CSR restricted to columns
+// [c0, c1)): the slice of t it reads stays resident in every XCD's L2.
+// Blocks after the first accumulate into `out`; the last one applies the
+// epilogue and produces the fused partial sums:
 //   kModeHv       : y = sum + delta v_i          partials (v.y, v.v, y.y)
 //   kModeResidual : y = b_i - (sum + delta v_i)  partials (y.y, |y - rref|^2)
 //   kModeRhs      : y = b_i - sum                partials (y.y)
-template <int G, class TV, int MODE>
+template <int G, int U, class TV, int MODE, bool FIRST, bool LAST>
 __global__ void __launch_bounds__(kBlock)
 k_csr(int m, const int* __restrict__ ptr, const int* __restrict__ idx, const TV* __restrict__ val,
       const double* __restrict__ t, const double* __restrict__ v, double delta,
       const double* __restrict__ bvec, const double* __restrict__ rref, double* __restrict__ out,
       double* __restrict__ part) {
   const int lane = threadIdx.x % G;
-  const int groups_per_block = kBlock / G;
+  const int gpb = kBlock / G;
+  const int g = threadIdx.x / G;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int64_t row = (int64_t)blockIdx.x * groups_per_block + threadIdx.x / G; row < m;
-       row += (int64_t)gridDim.x * groups_per_block) {
-    const int b = ptr[row], e = ptr[row + 1];
-    double s = 0.0;
-    for (int p = b + lane; p < e; p += G) s += (double)val[p] * t[idx[p]];
+  for (int64_t tile = (int64_t)blockIdx.x * U * gpb; tile < m; tile += (int64_t)gridDim.x * U * gpb) {
+    int b[U], e[U];
 #pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    for (int u = 0; u < U; ++u) {
+      const int64_t row = tile + (int64_t)u * gpb + g;
+      b[u] = row < m ? ptr[row] : 0;
+      e[u] = row < m ? ptr[row + 1] : 0;
+    }
+    int ix[U];
+    double a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int p = b[u] + lane;
+      const bool ok = p < e[u];
+      ix[u] = ok ? idx[p] : 0;
+      a[u] = ok ? (double)val[p] : 0.0;
+    }
+    double sum[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) sum[u] = a[u] * t[ix[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      for (int p = b[u] + lane + G; p < e[u]; p += G) sum[u] += (double)val[p] * t[idx[p]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) sum[u] += __shfl_xor(sum[u], o, 64);
+    }
     if (lane == 0) {
-      if (MODE == kModeHv) {
-        double vi = v[row];
-        double y = s + delta * vi;
-        out[row] = y;
-        acc[0] += vi * y;
-        acc[1] += vi * vi;
-        acc[2] += y * y;
-      } else if (MODE == kModeResidual) {
-        double y = bvec[row] - (s + delta * v[row]);
-        out[row] = y;
-        acc[0] += y * y;
-        if (rref) {
-          double d = y - rref[row];
-          acc[1] += d * d;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t row = tile + (int64_t)u * gpb + g;
+        if (row >= m) continue;
+        double sm = sum[u];
+        if (!FIRST) sm += out[row];
+        if (!LAST) {
+          out[row] = sm;
+        } else if (MODE == kModeHv) {
+          const double vi = v[row];
+          const double y = sm + delta * vi;
+          out[row] = y;
+          acc[0] += vi * y;
+          acc[1] += vi * vi;
+          acc[2] += y * y;
+        } else if (MODE == kModeResidual) {
+          const double y = bvec[row] - (sm + delta * v[row]);
+          out[row] = y;
+          acc[0] += y * y;
+          if (rref) {
+            const double d = y - rref[row];
+            acc[1] += d * d;
+          }
+        } else {
+          const double y = bvec[row] - sm;
+          out[row] = y;
+          acc[0] += y * y;
         }
-      } else {
-        double y = bvec[row] - s;
-        out[row] = y;
-        acc[0] += y * y;
       }
     }
   }
-  if (part) write_partials<3>(acc, part);
+  if (LAST && part) write_partials<3>(acc, part);
+}
+
+// one block: reduce NP per-block partials to NP scalars (fixed order)
+template <int NP>
+__global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  double tot[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) tot[k] = 0.0;
+  for (int b0 = 0; b0 < nb; b0 += 4 * kBlock) {
+    double v[NP][4];
+#pragma unroll
+    for (int k = 0; k < NP; ++k)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int b = b0 + q * kBlock + threadIdx.x;
+        v[k][q] = b < nb ? part[(size_t)k * nb + b] : 0.0;
+      }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) tot[k] += (v[k][0] + v[k][1]) + (v[k][2] + v[k][3]);
+  }
+  block_sum<NP>(tot);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NP; ++k) out[k] = tot[k];
+  }
 }
 
 __global__ void k_recip(int n, const double* __restrict__ a, double* __restrict__ out) {
@@ -124,12 +211,11 @@ k_dot(int n, const double* __restrict__ a, const double* __restrict__ b, double*
 // x += alpha p, r -= alpha q and partials of r.r.  Every block reduces the
 // same partials, so every block computes the same alpha.
 __global__ void __launch_bounds__(kBlock)
-k_pcg_step(int m, const double* __restrict__ partA, int nbA, const double* __restrict__ rz_cur,
+k_pcg_step(int m, const double* __restrict__ hv_dots, const double* __restrict__ rz_cur,
            const double* __restrict__ p, const double* __restrict__ q, double* __restrict__ x,
            double* __restrict__ r, double* __restrict__ scal, double* __restrict__ alpha_out) {
   if (scal[kStop] != 0.0) return;
-  double tot[3];
-  reduce_partials<3>(partA, nbA, tot);
+  const double tot[3] = {hv_dots[0], hv_dots[1], hv_dots[2]};
   const double curvature = tot[0];
   const double floor_ = std::numeric_limits<double>::min() * fmax(1.0, sqrt(tot[1]) * sqrt(tot[2]));
   const double alpha = *rz_cur / curvature;
@@ -199,12 +285,14 @@ k_pcg_direction(int m, const double* __restrict__ partC, int nbC, const double* 
 
 // d_full_i = sum_j a_ij^2 w_j [full_mask_j], d_core_i = same over core_mask (SAP:379-390)
 __global__ void __launch_bounds__(kBlock)
-k_diag_terms(int m, const int* __restrict__ ptr, const int* __restrict__ idx,
+k_diag_terms(int m, int nsplit, const int* __restrict__ ptr, const int* __restrict__ idx,
              const double* __restrict__ val, const double* __restrict__ w,
              const uint8_t* __restrict__ core_mask, double* __restrict__ full, double* __restrict__ core) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
     double f = 0.0, c = 0.0;
-    for (int p = ptr[i]; p < ptr[i + 1]; ++p) {
+    // entries of row i in increasing column order across the column blocks
+    for (int k = 0; k < nsplit; ++k)
+    for (int p = ptr[(size_t)k * (m + 1) + i]; p < ptr[(size_t)k * (m + 1) + i + 1]; ++p) {
       const int j = idx[p];
       const double a = val[p];
       const double t = a * a * w[j];
@@ -239,6 +327,8 @@ k_scores(int ncand, const int* __restrict__ cand, const int* __restrict__ ptr,
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
+constexpr int kUnroll = 1;  // segments per group per pass (random gathers are L2-line bound; 1 measured best)
+
 static int pick_group(double avg) {
   int g = 1;
   while (g < 64 && g < avg) g <<= 1;
@@ -249,9 +339,9 @@ template <class TV>
 static void launch_gather(int G, int nseg, const int* ptr, const int* idx, const TV* val,
                           const double* v, const double* scale, const double* add, double* out,
                           hipStream_t s) {
-  const int grid = grid_for((int64_t)nseg * G, kBlock, 65535);
+  const int grid = grid_for((int64_t)nseg * G, kBlock * kUnroll, 65535);
   switch (G) {
-#define CASE(g) case g: hipLaunchKernelGGL((k_seg_gather<g, TV>), dim3(grid), dim3(kBlock), 0, s, nseg, ptr, idx, val, v, scale, add, out); break;
+#define CASE(g) case g: hipLaunchKernelGGL((k_seg_gather<g, kUnroll, TV>), dim3(grid), dim3(kBlock), 0, s, nseg, ptr, idx, val, v, scale, add, out); break;
     CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
 #undef CASE
     default: throw Error(PSG_ERR_INTERNAL, "bad group size");
@@ -259,13 +349,13 @@ static void launch_gather(int G, int nseg, const int* ptr, const int* idx, const
   HIP_LAUNCH_CHECK();
 }
 
-template <class TV, int MODE>
-static int launch_csr(int G, int m, const int* ptr, const int* idx, const TV* val, const double* t,
-                      const double* v, double delta, const double* b, const double* rref,
-                      double* out, double* part, hipStream_t s) {
-  const int grid = grid_for((int64_t)m * G, kBlock, kRedBlocks);
+template <class TV, int MODE, bool FIRST, bool LAST>
+static int launch_csr_block(int G, int m, const int* ptr, const int* idx, const TV* val, const double* t,
+                            const double* v, double delta, const double* b, const double* rref, double* out,
+                            double* part, hipStream_t s) {
+  const int grid = grid_for((int64_t)m * G, kBlock * kUnroll, kSpmvBlocks);
   switch (G) {
-#define CASE(g) case g: hipLaunchKernelGGL((k_csr<g, TV, MODE>), dim3(grid), dim3(kBlock), 0, s, m, ptr, idx, val, t, v, delta, b, rref, out, part); break;
+#define CASE(g) case g: hipLaunchKernelGGL((k_csr<g, kUnroll, TV, MODE, FIRST, LAST>), dim3(grid), dim3(kBlock), 0, s, m, ptr, idx, val, t, v, delta, b, rref, out, part); break;
     CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
 #undef CASE
     default: throw Error(PSG_ERR_INTERNAL, "bad group size");
@@ -274,25 +364,53 @@ static int launch_csr(int G, int m, const int* ptr, const int* idx, const TV* va
   return grid;
 }
 
+// y = A t with the MODE epilogue, over the column blocks of the split CSR.
+// Returns the partials grid of the last launch.
+template <class TV, int MODE>
+static int launch_csr(const psg_ctx* c, const TV* val, const double* t, const double* v, double delta,
+                      const double* b, const double* rref, double* out, double* part, hipStream_t s) {
+  const int S = c->nsplit;
+  const int m = c->m;
+  int grid = 0;
+  for (int k = 0; k < S; ++k) {
+    const int* ptr = c->rptr.p + (size_t)k * (m + 1);
+    const bool first = k == 0, last = k == S - 1;
+    if (first && last)
+      grid = launch_csr_block<TV, MODE, true, true>(c->g_csr, m, ptr, c->rcol.p, val, t, v, delta, b, rref, out, part, s);
+    else if (first)
+      grid = launch_csr_block<TV, MODE, true, false>(c->g_csr, m, ptr, c->rcol.p, val, t, v, delta, b, rref, out, part, s);
+    else if (last)
+      grid = launch_csr_block<TV, MODE, false, true>(c->g_csr, m, ptr, c->rcol.p, val, t, v, delta, b, rref, out, part, s);
+    else
+      grid = launch_csr_block<TV, MODE, false, false>(c->g_csr, m, ptr, c->rcol.p, val, t, v, delta, b, rref, out, part, s);
+  }
+  return grid;
+}
+
 // out = H v on device vectors; the FP32 copy of A is used only when lowp.
-// Returns the partials grid (partials in `part`: v.y, v.v, y.y).
-static int device_Hv(psg_ctx* c, const double* v, double* out, double* part, bool lowp) {
+// Returns the partials grid (partials in `part`: v.y, v.v, y.y); with
+// `dots` set, a one-block kernel also reduces them to dots[0..2].
+static int device_Hv(psg_ctx* c, const double* v, double* out, double* part, bool lowp, double* dots = nullptr) {
+  int nb;
   if (lowp && c->precision == PSG_MIXED) {
     launch_gather<float>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval32.p, v, c->w.p, nullptr, c->t_n.p, c->stream);
-    return launch_csr<float, kModeHv>(c->g_csr, c->m, c->rptr.p, c->rcol.p, c->rval32.p, c->t_n.p, v,
-                                      c->delta, nullptr, nullptr, out, part, c->stream);
+    nb = launch_csr<float, kModeHv>(c, c->rval32.p, c->t_n.p, v, c->delta, nullptr, nullptr, out, part, c->stream);
+  } else {
+    launch_gather<double>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval.p, v, c->w.p, nullptr, c->t_n.p, c->stream);
+    nb = launch_csr<double, kModeHv>(c, c->rval.p, c->t_n.p, v, c->delta, nullptr, nullptr, out, part, c->stream);
   }
-  launch_gather<double>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval.p, v, c->w.p, nullptr, c->t_n.p, c->stream);
-  return launch_csr<double, kModeHv>(c->g_csr, c->m, c->rptr.p, c->rcol.p, c->rval.p, c->t_n.p, v,
-                                     c->delta, nullptr, nullptr, out, part, c->stream);
+  if (dots && part) {
+    hipLaunchKernelGGL(k_finalize<3>, dim3(1), dim3(kBlock), 0, c->stream, part, nb, dots);
+    HIP_LAUNCH_CHECK();
+  }
+  return nb;
 }
 
 // out = b - H v (always FP64: the outer residual); partials (|out|^2, |out - rref|^2)
 static int device_residual(psg_ctx* c, const double* b, const double* v, const double* rref,
                            double* out, double* part) {
   launch_gather<double>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval.p, v, c->w.p, nullptr, c->t_n.p, c->stream);
-  return launch_csr<double, kModeResidual>(c->g_csr, c->m, c->rptr.p, c->rcol.p, c->rval.p, c->t_n.p, v,
-                                           c->delta, b, rref, out, part, c->stream);
+  return launch_csr<double, kModeResidual>(c, c->rval.p, c->t_n.p, v, c->delta, b, rref, out, part, c->stream);
 }
 
 static int device_dot(psg_ctx* c, const double* a, const double* b, int n, double* part) {
@@ -315,8 +433,8 @@ static void copy_dev(psg_ctx* c, double* dst, const double* src, int n) {
 
 void diag_terms_device(psg_ctx* c, const uint8_t* mask_dev, double* full, double* core) {
   const int grid = grid_for(c->m, kBlock, 4096);
-  hipLaunchKernelGGL(k_diag_terms, dim3(grid), dim3(kBlock), 0, c->stream, c->m, c->rptr.p, c->rcol.p,
-                     c->rval.p, c->w.p, mask_dev, full, core);
+  hipLaunchKernelGGL(k_diag_terms, dim3(grid), dim3(kBlock), 0, c->stream, c->m, c->nsplit, c->rptr.p,
+                     c->rcol.p, c->rval.p, c->w.p, mask_dev, full, core);
   HIP_LAUNCH_CHECK();
 }
 
@@ -371,21 +489,39 @@ int psg_create(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowin
       HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
       HIP_CHECK(hipEventCreate(&c->ev0));
       HIP_CHECK(hipEventCreate(&c->ev1));
-      // CSC (int32) and its transpose (CSR) built on the host, both uploaded
+      // CSC (int32) and a column-blocked CSR (rows of block k cover columns
+      // [k n/S, (k+1) n/S)), both built on the host and uploaded; S is chosen
+      // so one block's slice of the n-vector t is <= 2 MiB (one XCD's L2 share)
       std::vector<int> cptr(n + 1), crow(rowind, rowind + nnz);
       for (int64_t j = 0; j <= n; ++j) cptr[j] = (int)colptr[j];
-      std::vector<int> rptr(m + 1, 0), rcol(nnz);
+      int S = (int)std::min<int64_t>(16, std::max<int64_t>(1, (n * 8 + (1 << 22) - 1) >> 22));
+      if (const char* e = getenv("PSG_HV_SPLIT")) S = std::max(1, std::min(64, atoi(e)));
+      c->nsplit = S;
+      std::vector<int> rptr((size_t)S * (m + 1), 0), rcol(nnz);
       std::vector<double> rval(nnz);
-      for (int64_t p = 0; p < nnz; ++p) rptr[rowind[p] + 1]++;
-      for (int64_t i = 0; i < m; ++i) rptr[i + 1] += rptr[i];
+      auto block_of = [&](int64_t j) { return (int)((j * S) / n); };
+      std::vector<int64_t> cnt((size_t)S * m, 0);
+      for (int64_t j = 0; j < n; ++j)
+        for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) cnt[(size_t)block_of(j) * m + rowind[p]]++;
       {
-        std::vector<int> fill(rptr.begin(), rptr.end() - 1);
-        for (int64_t j = 0; j < n; ++j)
+        int64_t off = 0;
+        for (int k = 0; k < S; ++k) {
+          for (int64_t i = 0; i < m; ++i) {
+            rptr[(size_t)k * (m + 1) + i] = (int)off;
+            off += cnt[(size_t)k * m + i];
+          }
+          rptr[(size_t)k * (m + 1) + m] = (int)off;
+        }
+        std::vector<int> fill(rptr.size());
+        for (size_t q = 0; q < rptr.size(); ++q) fill[q] = rptr[q];
+        for (int64_t j = 0; j < n; ++j) {
+          const size_t kb = (size_t)block_of(j) * (m + 1);
           for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) {
-            int q = fill[rowind[p]]++;
+            int q = fill[kb + rowind[p]]++;
             rcol[q] = (int)j;
             rval[q] = values[p];
           }
+        }
       }
       c->cptr.upload(cptr, c->stream);
       c->crow.upload(crow, c->stream);
@@ -398,8 +534,10 @@ int psg_create(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowin
         c->cval32.upload(cv, c->stream);
         c->rval32.upload(rv, c->stream);
       }
-      c->g_csc = pick_group((double)nnz / (double)n);
-      c->g_csr = pick_group((double)nnz / (double)m);
+      c->g_csc = pick_group(0.5 * (double)nnz / (double)n);
+      c->g_csr = pick_group(0.5 * (double)nnz / ((double)m * S));
+      if (const char* e = getenv("PSG_G_CSC")) c->g_csc = pick_group(atof(e));
+      if (const char* e = getenv("PSG_G_CSR")) c->g_csr = pick_group(atof(e));
       c->theta.alloc(n);
       c->w.alloc(n);
       c->t_n.alloc(n);
@@ -408,9 +546,10 @@ int psg_create(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowin
       for (auto* b : {&c->rhs, &c->x, &c->r, &c->z, &c->p, &c->q, &c->rt, &c->vin, &c->vout,
                       &c->d_full, &c->d_core, &c->dsc})
         b->alloc(m);
-      c->partA.alloc(3 * kRedBlocks);
+      c->partA.alloc(3 * kSpmvBlocks);
       c->partC.alloc(3 * kRedBlocks);
-      c->partD.alloc(3 * kRedBlocks);
+      c->partD.alloc(3 * kSpmvBlocks);
+      c->hvdots.alloc(4);
       c->scal.alloc(kScalarHeader + kRedBlocks);
       HIP_CHECK(hipHostMalloc((void**)&c->hscal, (kScalarHeader + kRedBlocks) * sizeof(double)));
       HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -522,8 +661,8 @@ int psg_solve_begin(psg_ctx* c, const double* r1, const double* r2) {
     hipLaunchKernelGGL(k_mul, dim3(grid_for(c->n, kBlock, 4096)), dim3(kBlock), 0, c->stream, c->n, c->w.p,
                        c->r1_n.p, c->t_n.p);
     HIP_LAUNCH_CHECK();
-    launch_csr<double, kModeRhs>(c->g_csr, c->m, c->rptr.p, c->rcol.p, c->rval.p, c->t_n.p, nullptr, 0.0,
-                                 c->vin.p, nullptr, c->rhs.p, nullptr, c->stream);
+    launch_csr<double, kModeRhs>(c, c->rval.p, c->t_n.p, nullptr, 0.0, c->vin.p, nullptr, c->rhs.p, nullptr,
+                                 c->stream);
     HIP_CHECK(hipStreamSynchronize(c->stream));
     c->have_rhs = true;
     c->have_solution = false;
@@ -567,8 +706,8 @@ int psg_pcg(psg_ctx* c, double tol, int32_t maxit, int32_t replacement, double g
     int replacements = 0, checks = 0;
     const int grid_m = grid_for(m, kBlock, kRedBlocks);
     while (info > 0 && it < maxit) {
-      const int nbA = device_Hv(c, c->p.p, c->q.p, c->partA.p, true);
-      hipLaunchKernelGGL(k_pcg_step, dim3(grid_m), dim3(kBlock), 0, s, m, c->partA.p, nbA, c->rzs.p + rz_slot,
+      device_Hv(c, c->p.p, c->q.p, c->partA.p, true, c->hvdots.p);
+      hipLaunchKernelGGL(k_pcg_step, dim3(grid_m), dim3(kBlock), 0, s, m, c->hvdots.p, c->rzs.p + rz_slot,
                          c->p.p, c->q.p, c->x.p, c->r.p, c->scal.p, c->alphas.p + it);
       HIP_LAUNCH_CHECK();
       HIP_CHECK(hipMemcpyAsync(c->hscal, c->scal.p, (kScalarHeader + grid_m) * sizeof(double),
@@ -695,8 +834,8 @@ int psg_time_kernels(psg_ctx* c, int32_t reps, double* out) {
       HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
       return (double)ms / reps;
     };
-    out[0] = timed([&] { device_Hv(c, c->vin.p, c->vout.p, c->partA.p, false); });
-    out[1] = timed([&] { device_Hv(c, c->vin.p, c->vout.p, c->partA.p, true); });
+    out[0] = timed([&] { device_Hv(c, c->vin.p, c->vout.p, c->partA.p, false, c->hvdots.p); });
+    out[1] = timed([&] { device_Hv(c, c->vin.p, c->vout.p, c->partA.p, true, c->hvdots.p); });
     out[2] = factor_ready(c) ? timed([&] { factor_apply(c, c->vin.p, c->vout.p); }) : -1.0;
     out[3] = timed([&] { device_dot(c, c->vin.p, c->vout.p, c->m, c->partD.p); });
     return PSG_OK;

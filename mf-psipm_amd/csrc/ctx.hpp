@@ -31,6 +31,7 @@ struct psg_ctx {
   psg::DevBuf<double> cval, rval;
   psg::DevBuf<float> cval32, rval32;
   int g_csc = 8, g_csr = 32;  // lanes per column / per row
+  int nsplit = 1;             // column blocks of the CSR copy (rptr holds nsplit x (m+1))
   // Newton matrix
   psg::DevBuf<double> theta, w;
   double delta = 0.0;
@@ -45,6 +46,7 @@ struct psg_ctx {
   psg::DevBuf<double> cand_out;
   // reductions and PCG scalars
   psg::DevBuf<double> partA, partC, partD;  // per-block partials
+  psg::DevBuf<double> hvdots;               // p.q, p.p, q.q of the last H p
   psg::DevBuf<double> scal;                 // header + partials of the axpy kernel
   psg::DevBuf<double> rzs, alphas, betas;
   double* hscal = nullptr;                  // pinned mirror of scal

@@ -236,3 +236,23 @@ def test_single_launch_solve_matches_level_scheduled(gpu, monkeypatch, precision
         apply_ref, cond = _dense_precond(A, core, 1.0 / theta, 1e-2)
         ref = apply_ref(r)
         assert np.linalg.norm(s1 - ref) <= (1e-9 if precision == "fp64" else 5e-7 * cond) * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("nsplit", [2, 3, 7])
+def test_apply_H_column_blocked_csr(gpu, monkeypatch, nsplit):
+    _say(f"apply_H split={nsplit}")
+    monkeypatch.setenv("PSG_HV_SPLIT", str(nsplit))
+    _, lp = _lp(2000, 6000, 5, 61)
+    obj = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos)
+    theta = _theta(6000, 62)
+    obj._call("psg_set_weights", obj._ctx, theta, 1e-4)
+    v = np.random.default_rng(63).standard_normal(2000)
+    ref = lp.A @ ((1.0 / theta) * (lp.A.T @ v)) + 1e-4 * v
+    assert np.abs(obj._full_matvec(v) - ref).max() <= 1e-12 * np.abs(ref).max()
+    w = 1.0 / theta
+    d_full = np.empty(2000)
+    obj._call("psg_diag_terms", obj._ctx, None, d_full, None)
+    assert np.allclose(d_full, lp.A.multiply(lp.A) @ w, rtol=1e-12)
+    r1, r2 = np.random.default_rng(64).standard_normal(6000), np.random.default_rng(65).standard_normal(2000)
+    obj._call("psg_solve_begin", obj._ctx, r1, r2)
+    obj.close()

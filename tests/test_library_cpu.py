@@ -1,0 +1,103 @@
+"""CPU tests of the C-ABI library: loading, exported symbols, host symbolic
+analysis (checked by a numpy executor of the plan against a dense Cholesky)
+and error paths.  No GPU compute is called here."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import mf_psipm_amd as P
+from mf_psipm_amd import _lib
+from mf_psipm_amd.symbolic import SymbolicPlan
+import plan_executor as PE
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "psipm_hip.h"
+
+
+def test_library_exports_every_declared_symbol(lib):
+    text = HEADER.read_text()
+    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(psg_\w+)\s*\(", text, flags=re.M))
+    assert len(declared) >= 20
+    bound = {name for name, _, _ in _lib.SIGNATURES}
+    assert declared == bound
+    for name in declared:
+        assert getattr(lib, name) is not None
+
+
+def test_device_count_never_fails(lib):
+    assert _lib.device_count() >= 0
+
+
+def test_error_paths_return_status_and_message(lib):
+    handle = C.c_void_p()
+    st = lib.psg_plan_create(-1, 5, np.zeros(6, np.int64), np.zeros(1, np.int32), np.zeros(1, np.int32), 0, 1, 64,
+                             10.0, C.byref(handle))
+    assert st < 0 and b"invalid" in lib.psg_last_error()
+    A = sp.random(10, 20, density=0.3, random_state=0, format="csc")
+    bad_core = np.array([25], np.int32)
+    st = lib.psg_plan_create(10, 20, A.indptr.astype(np.int64), A.indices.astype(np.int32), bad_core, 1, 1, 64,
+                             10.0, C.byref(handle))
+    assert st < 0 and b"out of range" in lib.psg_last_error()
+    if _lib.device_count() == 0:
+        ctx = C.c_void_p()
+        st = lib.psg_create(10, 20, A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data, 0, C.byref(ctx))
+        assert st == -2 and b"no GPU" in lib.psg_last_error()
+
+
+def _check_plan(A, core, **kw):
+    m = A.shape[0]
+    plan = SymbolicPlan(A, core, **kw)
+    ex = plan.export()
+    assert sorted(ex["perm"].tolist()) == list(range(m))
+    rng = np.random.default_rng(0)
+    w = 10.0 ** rng.uniform(-2, 2, A.shape[1])
+    Ac = A[:, core]
+    M = (Ac @ sp.diags(w[core]) @ Ac.T).toarray() + 0.5 * np.eye(m)
+    diag = np.diag(M).copy()
+    panels = PE.assemble_panels(ex, A.tocsc(), w, diag)
+    assert PE.factor_panels(ex, panels) == -1
+    L = PE.panels_to_dense_L(ex, panels)
+    dsc = 1.0 / np.sqrt(diag)
+    Ms = (dsc[:, None] * M * dsc[None, :])[np.ix_(ex["perm"], ex["perm"])]
+    assert np.abs(L @ L.T - Ms).max() <= 1e-12
+    ref = np.linalg.cholesky(Ms)
+    assert plan.info["nnz_L"] >= np.count_nonzero(np.abs(ref) > 1e-14)
+    return plan
+
+
+@pytest.mark.parametrize("ordering,width,frac", [(1, 64, 1.0), (1, 16, 0.5), (0, 8, 0.8), (1, 1, 0.3)])
+def test_plan_reproduces_dense_cholesky_random(ordering, width, frac):
+    raw, *_ = P.generators.random_lp(400, 1200, 5, 1)
+    A = raw.A.tocsc()
+    core = np.sort(np.random.default_rng(2).choice(1200, int(frac * 400), replace=False))
+    plan = _check_plan(A, core, ordering=ordering, max_width=width)
+    assert plan.info["max_width"] <= width
+
+
+def test_plan_reproduces_dense_cholesky_network():
+    lp = P.generators.grid_min_cost_flow(12, 12, 6, 3)
+    lp, _ = P.lp.ground_network(lp)
+    core = np.arange(0, lp.A.shape[1], 2)
+    _check_plan(lp.A.tocsc(), core, ordering=1, max_width=32)
+
+
+def test_minimum_degree_reduces_fill_and_plan_is_deterministic():
+    raw, *_ = P.generators.random_lp(1500, 4500, 5, 4)
+    A = raw.A.tocsc()
+    core = np.sort(np.random.default_rng(5).choice(4500, 1200, replace=False))
+    natural = SymbolicPlan(A, core, ordering=0).info["nnz_L"]
+    md = SymbolicPlan(A, core, ordering=1)
+    assert md.info["nnz_L"] < 0.6 * natural
+    again = SymbolicPlan(A, core, ordering=1)
+    e1, e2 = md.export(), again.export()
+    for k in ("perm", "sfirst", "rows", "seg_target"):
+        assert np.array_equal(e1[k], e2[k])
+
+
+def test_plan_of_empty_core_is_diagonal():
+    raw, *_ = P.generators.random_lp(50, 100, 5, 6)
+    plan = SymbolicPlan(raw.A.tocsc(), np.empty(0, np.int64))
+    assert plan.info["nnz_L"] == 50 and plan.info["npairs"] == 0
