@@ -114,7 +114,8 @@ class HipSpectralStableColumnPreconditioner:
         corrector_tightening=1.0, minres_fallback=True, minres_maxit=400,
         minres_max_dimension=200_000, fail_tol=1e-6, low_rank_max_columns=8,
         low_rank_min_nnz=1024, low_rank_clique_ratio=2.0, ordering=1, max_width=64,
-        dense_factor=10.0, max_factor_entries: Optional[float] = None, **_ignored,
+        dense_factor=10.0, max_factor_entries: Optional[float] = None, error_class=KKTSolveError,
+        **_ignored,
     ):
         if core_factor <= 0.0:
             raise ValueError("core_factor must be positive")
@@ -133,6 +134,9 @@ class HipSpectralStableColumnPreconditioner:
         if weighted_backbone:
             raise NotImplementedError("the weighted matching backbone is not part of this build")
         self._lib = _lib.load()
+        # exception raised where the reference raises KKTSolveError (SSP:733, 743);
+        # the reference driver passes its own class so prox_eval catches it
+        self.error_class = error_class
         self.A = A.tocsc()
         self.A.sum_duplicates()
         self.A.sort_indices()
@@ -445,7 +449,7 @@ class HipSpectralStableColumnPreconditioner:
         else:
             self._have_factor = False
             self.t_fact += time.perf_counter() - t0
-            raise KKTSolveError("Cholesky failed even with a 1e-2 shift")
+            raise self.error_class("Cholesky failed even with a 1e-2 shift")
         fms = C.c_double(0.0)
         self._call("psg_last_timings", self._ctx, None, C.byref(fms))
         self.device_factor_ms += fms.value
@@ -633,12 +637,12 @@ class HipSpectralStableColumnPreconditioner:
             self.full_fallbacks += 1
             if not ok:
                 self.t_solve += max(0.0, time.perf_counter() - started - (self.t_fact - fact_before))
-                raise KKTSolveError(f"spectral PCG and augmented MINRES failed: pcg info={res.info}, "
+                raise self.error_class(f"spectral PCG and augmented MINRES failed: pcg info={res.info}, "
                                     f"pcg residual={res.relative_residual:.2e}, minres residual={rel:.2e}")
         else:
             self.cg_failures += 1
             self.t_solve += max(0.0, time.perf_counter() - started - (self.t_fact - fact_before))
-            raise KKTSolveError(f"spectral stable-column PCG failed: info={res.info}, "
+            raise self.error_class(f"spectral stable-column PCG failed: info={res.info}, "
                                 f"residual={res.relative_residual:.2e}")
         self.n_solve += 1
         self.t_solve += max(0.0, time.perf_counter() - started - (self.t_fact - fact_before))
