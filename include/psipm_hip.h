@@ -146,6 +146,11 @@ int psg_time_kernels(psg_ctx* ctx, int32_t reps, double* out);
  * environment variable PSG_SF_TRACE set; *nunits = 0 otherwise.  No
  * counterpart in the reference. */
 int psg_solve_trace(psg_ctx* ctx, int64_t* out, int64_t cap, int64_t* nunits);
+/* Diagnostics of the factor's solve layout, out[6] = {solve mode (0 level
+ * scheduled, 1 single launch), dependency-driven units, isolated 1x1
+ * supernodes, dense-tail order r (0: none), dense-tail supernodes, nsuper}.
+ * No counterpart in the reference. */
+int psg_factor_info(psg_ctx* ctx, double* out);
 /* device time (ms, HIP events) of the last psg_pcg and last psg_factor calls */
 int psg_last_timings(psg_ctx* ctx, double* pcg_ms, double* factor_ms);
 

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <climits>
 #include <numeric>
+#include <tuple>
 #include <cmath>
 
 #include "ctx.hpp"
@@ -40,6 +41,15 @@ struct LevelRange {
   int unit_begin, unit_end;   // TRSM units (below rows only)
   int sunit_begin, sunit_end; // solve units (>= 1 per supernode)
   int tile_begin, tile_end;   // update tiles
+};
+
+// one product C = alpha A B of the tail inversion (column-major blocks)
+struct GemmDesc {
+  int M, N, K;
+  int64_t a_off, b_off, c_off;  // element offsets into the A, B, C base buffers
+  int lda, ldb, ldc;
+  int a_sel, b_sel, c_sel;      // 0: Ld, 1: X, 2: scratch
+  double alpha;
 };
 
 struct FactorState {
@@ -66,6 +76,16 @@ struct FactorState {
   int sf_units = 0;
   DevBuf<int> su_s, su_r0, su_n, su_range, need, flags, bwd_order, iso_col, iso_doff;
   int n_iso = 0;
+  // dense tail [tail_F, tail_E) = supernodes [tail_s0, tail_s0 + tail_nchunk)
+  int tail_r = 0, tail_F = 0, tail_s0 = 0, tail_nchunk = 0;
+  DevBuf<int> tail_chunk_of, tail_lower_tiles, tail_upper_tiles;
+  int tail_nlower = 0, tail_nupper = 0;
+  DevBuf<double> tail_Ld64, tail_X64, tail_S64;
+  DevBuf<float> tail_Ld32, tail_X32, tail_S32;
+  struct GemmLevel { int desc_begin, ndesc, tile_begin, ntile; };
+  std::vector<GemmLevel> gemm_levels;  // two launches per level (T = B Ai, then X21 = -Ci T)
+  DevBuf<GemmDesc> gemm_descs;
+  DevBuf<int> gemm_tile_desc, gemm_tile_mn;
   DevBuf<long long> trace;  // allocated when PSG_SF_TRACE is set
   DevBuf<double> sf_part64;
   DevBuf<float> sf_part32;
@@ -742,9 +762,308 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// dense tail: the top of the tree is a dense lower-triangular block L_T
+// (columns [F, E), every later column in every structure).  Its explicit
+// inverse X = L_T^-1 turns the tail of both triangular solves -- a chain of
+// ~r/64 dependent steps -- into one bandwidth-bound product each.
+// ---------------------------------------------------------------------------
+constexpr int kGT = 64;   // GEMM tile edge
+constexpr int kGK = 32;   // GEMM k-chunk
+
+// Ld (r x r, column-major) <- lower part of the tail panels, zeros above
+template <class T>
+__global__ void k_tail_gather(int r, int F, const int* __restrict__ chunk_of, const int* __restrict__ sfirst,
+                              const int64_t* __restrict__ rptr, const int64_t* __restrict__ poff,
+                              const T* __restrict__ panels, T* __restrict__ Ld) {
+  const int64_t total = (int64_t)r * r;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e / r), i = (int)(e % r);
+    T v = T(0);
+    if (i >= c) {
+      const int s = chunk_of[c];
+      const int cf = sfirst[s] - F;
+      const int64_t nr = rptr[s + 1] - rptr[s];
+      v = panels[poff[s] + (int64_t)(c - cf) * nr + (i - cf)];
+    }
+    Ld[e] = v;
+  }
+}
+
+// X diagonal blocks <- stored block inverses
+template <class T>
+__global__ void k_tail_diag(int r, int F, int nchunk, int s0, const int* __restrict__ sfirst,
+                            const int64_t* __restrict__ doff, const T* __restrict__ dinv, T* __restrict__ X) {
+  const int k = blockIdx.x;
+  if (k >= nchunk) return;
+  const int s = s0 + k;
+  const int cf = sfirst[s] - F;
+  const int w = sfirst[s + 1] - sfirst[s];
+  const T* D = dinv + doff[s];
+  for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
+    const int c = idx / w, a = idx % w;
+    X[(int64_t)(cf + c) * r + (cf + a)] = D[idx];
+  }
+}
+
+
+// Batched C = alpha A B (column-major, beta = 0) over 64x64 output tiles.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_gemm_batched(const GemmDesc* __restrict__ descs, const int* __restrict__ tile_desc,
+               const int* __restrict__ tile_mn, T* __restrict__ Ld, T* __restrict__ X, T* __restrict__ Sc) {
+  __shared__ T sA[kGK][kGT + 1];
+  __shared__ T sB[kGK][kGT + 1];
+  const GemmDesc d = descs[tile_desc[blockIdx.x]];
+  const int tm = tile_mn[2 * blockIdx.x], tn = tile_mn[2 * blockIdx.x + 1];
+  T* bases[3] = {Ld, X, Sc};
+  const T* A = bases[d.a_sel] + d.a_off;
+  const T* B = bases[d.b_sel] + d.b_off;
+  T* C = bases[d.c_sel] + d.c_off;
+  const int tid = threadIdx.x;
+  const int tx = tid % 16, ty = tid / 16;
+  const int i0 = tm * kGT, j0 = tn * kGT;
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+  for (int k0 = 0; k0 < d.K; k0 += kGK) {
+    __syncthreads();
+    for (int idx = tid; idx < kGK * kGT; idx += blockDim.x) {
+      {  // A(gi, gk): consecutive lanes walk a column of A
+        const int kk = idx / kGT, a = idx % kGT;
+        const int gi = i0 + a, gk = k0 + kk;
+        sA[kk][a] = (gi < d.M && gk < d.K) ? A[(int64_t)gk * d.lda + gi] : T(0);
+      }
+      {  // B(gk, gj): consecutive lanes walk a column of B
+        const int a = idx / kGK, kk = idx % kGK;
+        const int gj = j0 + a, gk = k0 + kk;
+        sB[kk][a] = (gj < d.N && gk < d.K) ? B[(int64_t)gj * d.ldb + gk] : T(0);
+      }
+    }
+    __syncthreads();
+    const int kc = min(kGK, d.K - k0);
+    for (int kk = 0; kk < kc; ++kk) {
+      T a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = sA[kk][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = sB[kk][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gi = i0 + ty * 4 + i, gj = j0 + tx * 4 + j;
+      if (gi < d.M && gj < d.N) C[(int64_t)gj * d.ldc + gi] = (T)(d.alpha * (double)acc[i][j]);
+    }
+}
+
+// y_T += X b_T over tiles (256 rows x 512 columns, lower part only)
+constexpr int kTR = 256, kTC = 512;
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_tail_lower(int r, const int* __restrict__ tiles, const T* __restrict__ X, const T* __restrict__ b,
+             T* __restrict__ y) {
+  __shared__ T bs[kTC];
+  const int I = tiles[2 * blockIdx.x], J = tiles[2 * blockIdx.x + 1];
+  const int j0 = J * kTC;
+  for (int q = threadIdx.x; q < kTC; q += blockDim.x) bs[q] = (j0 + q < r) ? b[j0 + q] : T(0);
+  __syncthreads();
+  const int i = I * kTR + threadIdx.x;
+  if (i >= r) return;
+  const int jend = min(min(j0 + kTC, r), i + 1);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  int j = j0;
+  for (; j + 3 < jend; j += 4) {
+    const T x0 = X[(int64_t)j * r + i], x1 = X[(int64_t)(j + 1) * r + i];
+    const T x2 = X[(int64_t)(j + 2) * r + i], x3 = X[(int64_t)(j + 3) * r + i];
+    acc[0] += (double)x0 * bs[j - j0];
+    acc[1] += (double)x1 * bs[j + 1 - j0];
+    acc[2] += (double)x2 * bs[j + 2 - j0];
+    acc[3] += (double)x3 * bs[j + 3 - j0];
+  }
+  for (; j < jend; ++j) acc[0] += (double)X[(int64_t)j * r + i] * bs[j - j0];
+  const double sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  if (jend > j0) __hip_atomic_fetch_add(y + i, (T)sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// x_T += X' y_T over tiles (64 columns x 1024 rows): a wave owns a column,
+// its lanes stream the column's contiguous rows
+constexpr int kUC = 64, kUR = 1024;
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_tail_upper(int r, const int* __restrict__ tiles, const T* __restrict__ X, const T* __restrict__ y,
+             T* __restrict__ x) {
+  __shared__ T ys[kUR];
+  const int J = tiles[2 * blockIdx.x], I = tiles[2 * blockIdx.x + 1];
+  const int i0 = I * kUR;
+  for (int q = threadIdx.x; q < kUR; q += blockDim.x) ys[q] = (i0 + q < r) ? y[i0 + q] : T(0);
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  for (int cc = wv; cc < kUC; cc += kBlock / kWave) {
+    const int j = J * kUC + cc;
+    if (j >= r) break;
+    const int ibeg = max(i0, j), iend = min(i0 + kUR, r);
+    if (ibeg >= iend) continue;
+    double acc = 0.0;
+    const T* col = X + (int64_t)j * r;
+    for (int i = ibeg + lane; i < iend; i += kWave) acc += (double)col[i] * ys[i - i0];
+    acc = wave_sum(acc);
+    if (lane == 0) __hip_atomic_fetch_add(x + j, (T)acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ void k_fill_int(int* __restrict__ p, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
+
+// Tail buffers, GEMV tile lists and the recursive inversion schedule.
+static void setup_tail(psg_ctx* c, const Plan& P) {
+  FactorState* fs = c->fs;
+  hipStream_t st = c->stream;
+  const int r = fs->tail_r, F = fs->tail_F;
+  std::vector<int> chunk_of(r);
+  std::vector<int> cbeg(fs->tail_nchunk + 1);
+  for (int k = 0; k < fs->tail_nchunk; ++k) {
+    const int sn = fs->tail_s0 + k;
+    cbeg[k] = P.sfirst[sn] - F;
+    for (int col = P.sfirst[sn]; col < P.sfirst[sn + 1]; ++col) chunk_of[col - F] = sn;
+  }
+  cbeg[fs->tail_nchunk] = r;
+  fs->tail_chunk_of.upload(chunk_of, st);
+  std::vector<int> lt, ut;
+  for (int I = 0; I * kTR < r; ++I)
+    for (int J = 0; J * kTC < std::min(r, (I + 1) * kTR); ++J) { lt.push_back(I); lt.push_back(J); }
+  for (int J = 0; J * kUC < r; ++J)
+    for (int I = (J * kUC) / kUR; I * kUR < r; ++I) { ut.push_back(J); ut.push_back(I); }
+  fs->tail_nlower = (int)lt.size() / 2;
+  fs->tail_nupper = (int)ut.size() / 2;
+  fs->tail_lower_tiles.upload(lt, st);
+  fs->tail_upper_tiles.upload(ut, st);
+  // recursive 2x2 inversion: pair neighbouring segments level by level;
+  // for pair (cols [a,b) | [b,c)):  S = L[b:c, a:b] X[a:b, a:b]  (scratch),
+  //                                 X[b:c, a:b] = -X[b:c, b:c] S
+  std::vector<std::pair<int, int>> segs;
+  for (int k = 0; k < fs->tail_nchunk; ++k) segs.emplace_back(cbeg[k], cbeg[k + 1]);
+  std::vector<GemmDesc> descs;
+  std::vector<int> tdesc, tmn;
+  fs->gemm_levels.clear();
+  int64_t scratch_need = 0;
+  auto add_gemm = [&](const GemmDesc& d) {
+    const int id = (int)descs.size();
+    descs.push_back(d);
+    for (int tm = 0; tm * kGT < d.M; ++tm)
+      for (int tn = 0; tn * kGT < d.N; ++tn) {
+        tdesc.push_back(id);
+        tmn.push_back(tm);
+        tmn.push_back(tn);
+      }
+  };
+  while (segs.size() > 1) {
+    std::vector<std::pair<int, int>> next;
+    FactorState::GemmLevel g1{(int)descs.size(), 0, (int)tdesc.size(), 0};
+    int64_t soff = 0;
+    std::vector<std::tuple<int, int, int, int64_t>> pairs;
+    for (size_t q = 0; q + 1 < segs.size(); q += 2) {
+      const int a = segs[q].first, b = segs[q].second, cc = segs[q + 1].second;
+      GemmDesc d{};
+      d.M = cc - b; d.N = b - a; d.K = b - a;
+      d.a_sel = 0; d.a_off = (int64_t)a * r + b; d.lda = r;        // L[b:c, a:b]
+      d.b_sel = 1; d.b_off = (int64_t)a * r + a; d.ldb = r;        // X[a:b, a:b]
+      d.c_sel = 2; d.c_off = soff; d.ldc = cc - b;                 // scratch
+      d.alpha = 1.0;
+      add_gemm(d);
+      pairs.emplace_back(a, b, cc, soff);
+      soff += (int64_t)(cc - b) * (b - a);
+      next.emplace_back(a, cc);
+    }
+    if (segs.size() % 2) next.push_back(segs.back());
+    g1.ndesc = (int)descs.size() - g1.desc_begin;
+    g1.ntile = (int)tdesc.size() - g1.tile_begin;
+    FactorState::GemmLevel g2{(int)descs.size(), 0, (int)tdesc.size(), 0};
+    for (auto& pr : pairs) {
+      const int a = std::get<0>(pr), b = std::get<1>(pr), cc = std::get<2>(pr);
+      const int64_t so = std::get<3>(pr);
+      GemmDesc d{};
+      d.M = cc - b; d.N = b - a; d.K = cc - b;
+      d.a_sel = 1; d.a_off = (int64_t)b * r + b; d.lda = r;        // X[b:c, b:c]
+      d.b_sel = 2; d.b_off = so; d.ldb = cc - b;                   // scratch
+      d.c_sel = 1; d.c_off = (int64_t)a * r + b; d.ldc = r;        // X[b:c, a:b]
+      d.alpha = -1.0;
+      add_gemm(d);
+    }
+    g2.ndesc = (int)descs.size() - g2.desc_begin;
+    g2.ntile = (int)tdesc.size() - g2.tile_begin;
+    fs->gemm_levels.push_back(g1);
+    fs->gemm_levels.push_back(g2);
+    scratch_need = std::max(scratch_need, soff);
+    segs.swap(next);
+  }
+  fs->gemm_descs.upload(descs, st);
+  fs->gemm_tile_desc.upload(tdesc, st);
+  fs->gemm_tile_mn.upload(tmn, st);
+  const size_t rr = (size_t)r * r;
+  if (c->precision == PSG_MIXED) {
+    fs->tail_Ld32.alloc(rr); fs->tail_X32.alloc(rr); fs->tail_S32.alloc(std::max<int64_t>(1, scratch_need));
+    fs->tail_Ld64.release(); fs->tail_X64.release(); fs->tail_S64.release();
+  } else {
+    fs->tail_Ld64.alloc(rr); fs->tail_X64.alloc(rr); fs->tail_S64.alloc(std::max<int64_t>(1, scratch_need));
+    fs->tail_Ld32.release(); fs->tail_X32.release(); fs->tail_S32.release();
+  }
+}
+
+template <class T>
+static void tail_invert(psg_ctx* c, const T* panels, const T* dinv, T* Ld, T* X, T* Sc) {
+  FactorState* fs = c->fs;
+  hipStream_t st = c->stream;
+  const int r = fs->tail_r;
+  const int64_t rr = (int64_t)r * r;
+  hipLaunchKernelGGL((k_tail_gather<T>), dim3(grid_for(rr, kBlock, 65535)), dim3(kBlock), 0, st, r, fs->tail_F,
+                     fs->tail_chunk_of.p, fs->sfirst.p, fs->rptr.p, fs->poff.p, panels, Ld);
+  HIP_LAUNCH_CHECK();
+  HIP_CHECK(hipMemsetAsync(X, 0, rr * sizeof(T), st));
+  hipLaunchKernelGGL((k_tail_diag<T>), dim3(fs->tail_nchunk), dim3(kBlock), 0, st, r, fs->tail_F, fs->tail_nchunk,
+                     fs->tail_s0, fs->sfirst.p, fs->doff.p, dinv, X);
+  HIP_LAUNCH_CHECK();
+  for (const auto& g : fs->gemm_levels) {
+    if (!g.ntile) continue;
+    hipLaunchKernelGGL((k_gemm_batched<T>), dim3(g.ntile), dim3(kBlock), 0, st, fs->gemm_descs.p,
+                       fs->gemm_tile_desc.p + g.tile_begin, fs->gemm_tile_mn.p + 2 * (int64_t)g.tile_begin, Ld, X, Sc);
+    HIP_LAUNCH_CHECK();
+  }
+}
+
+template <class T>
+static void tail_solve(psg_ctx* c, const T* X, T* vb, T* vy, T* vx) {
+  FactorState* fs = c->fs;
+  hipStream_t st = c->stream;
+  const int r = fs->tail_r, F = fs->tail_F;
+  HIP_CHECK(hipMemsetAsync(vy + F, 0, (size_t)r * sizeof(T), st));
+  hipLaunchKernelGGL((k_tail_lower<T>), dim3(fs->tail_nlower), dim3(kBlock), 0, st, r, fs->tail_lower_tiles.p, X,
+                     vb + F, vy + F);
+  HIP_LAUNCH_CHECK();
+  HIP_CHECK(hipMemsetAsync(vx + F, 0, (size_t)r * sizeof(T), st));
+  hipLaunchKernelGGL((k_tail_upper<T>), dim3(fs->tail_nupper), dim3(kBlock), 0, st, r, fs->tail_upper_tiles.p, X,
+                     vy + F, vx + F);
+  HIP_LAUNCH_CHECK();
+  // tail supernodes count as solved for the backward dependency waits
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_for(fs->tail_nchunk, kBlock, 1024)), dim3(kBlock), 0, st,
+                     fs->flags.p + 3 * fs->nsuper + fs->tail_s0, fs->tail_nchunk, 1);
+  HIP_LAUNCH_CHECK();
+}
+
 static void upload_plan(psg_ctx* c, const Plan& P) {
   FactorState* fs = c->fs;
   hipStream_t s = c->stream;
@@ -821,6 +1140,38 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
       if (P.sparent[sn] >= 0) level[P.sparent[sn]] = std::max(level[P.sparent[sn]], level[sn] + 1);
     for (int sn = P.nsuper - 1; sn >= 0; --sn)
       if (P.sparent[sn] >= 0) depth[sn] = depth[P.sparent[sn]] + 1;
+    // dense tail: the longest chain ending at a root whose structures are all
+    // "every later column" -- inverted explicitly when large enough
+    {
+      const char* tm = getenv("PSG_TAIL_MIN");
+      const int tail_min = tm ? atoi(tm) : 2048;
+      int best_q = -1, best_s0 = -1, best_r = 0;
+      for (int q = 0; q < P.nsuper; ++q) {
+        if (P.sparent[q] >= 0) continue;
+        const int E = P.sfirst[q + 1];
+        int s0 = q;
+        while (true) {
+          const int sn = s0;
+          const int64_t nr = P.rptr[sn + 1] - P.rptr[sn];
+          if (nr != E - P.sfirst[sn]) break;
+          if (sn != q && P.sparent[sn] != sn + 1) break;
+          if (sn == 0) { s0 = -1; break; }
+          s0 = sn - 1;
+        }
+        const int first = s0 + 1;
+        const int r = E - P.sfirst[first];
+        if (first <= q && r > best_r) { best_r = r; best_q = q; best_s0 = first; }
+      }
+      fs->tail_r = 0;
+      fs->tail_nchunk = 0;
+      if (best_q >= 0 && best_r >= tail_min) {
+        fs->tail_r = best_r;
+        fs->tail_s0 = best_s0;
+        fs->tail_nchunk = best_q - best_s0 + 1;
+        fs->tail_F = P.sfirst[best_s0];
+      }
+    }
+    auto in_tail = [&](int sn) { return fs->tail_nchunk && sn >= fs->tail_s0 && sn < fs->tail_s0 + fs->tail_nchunk; };
     // isolated 1x1 supernodes (rows no active column touches) are solved
     // elementwise, x = b / l_kk^2, outside the dependency-driven kernels
     std::vector<int> iso_col, iso_doff;
@@ -834,7 +1185,7 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
       if (iso) {
         iso_col.push_back(P.sfirst[sn]);
         iso_doff.push_back((int)doff[sn]);
-      } else {
+      } else if (!in_tail(sn)) {
         order.push_back(sn);
       }
     }
@@ -870,6 +1221,7 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
       range[2 * sn + 1] = (int)us.size();
     }
     fs->sf_units = (int)us.size();
+    if (fs->tail_nchunk) setup_tail(c, P);
     // backward: parents first by depth below the root
     std::vector<int> border(us.size());
     std::iota(border.begin(), border.end(), 0);
@@ -1008,6 +1360,12 @@ static int factor_run(psg_ctx* c, const Plan& P, double eta, T* panels, T* dinv)
   int failcol = INT_MAX;
   HIP_CHECK(hipMemcpyAsync(&failcol, fs->fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
+  if (failcol == INT_MAX && fs->tail_nchunk && fs->solve_mode == 1) {
+    if constexpr (sizeof(T) == 8)
+      tail_invert<double>(c, panels, dinv, fs->tail_Ld64.p, fs->tail_X64.p, fs->tail_S64.p);
+    else
+      tail_invert<float>(c, panels, dinv, fs->tail_Ld32.p, fs->tail_X32.p, fs->tail_S32.p);
+  }
   return failcol == INT_MAX ? 0 : failcol + 1;
 }
 
@@ -1116,6 +1474,12 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
     hipLaunchKernelGGL((k_fwd_sf<T>), dim3(grid), dim3(kBlock), 0, s, a);
     HIP_LAUNCH_CHECK();
   }
+  if (fs->tail_nchunk) {
+    if constexpr (sizeof(T) == 8)
+      tail_solve<double>(c, fs->tail_X64.p, vb, vy, vx);
+    else
+      tail_solve<float>(c, fs->tail_X32.p, vb, vy, vx);
+  }
   if (fs->sf_units) {
     hipLaunchKernelGGL((k_bwd_sf<T>), dim3(grid), dim3(kBlock), 0, s, a);
     HIP_LAUNCH_CHECK();
@@ -1194,6 +1558,21 @@ extern "C" int psg_solve_trace(psg_ctx* c, int64_t* out, int64_t cap, int64_t* n
   return guarded([&]() -> int {
     if (!c || !out || !nunits) throw Error(PSG_ERR_ARGUMENT, "psg_solve_trace: null argument");
     *nunits = factor_trace(c, reinterpret_cast<long long*>(out), cap);
+    return PSG_OK;
+  });
+}
+
+extern "C" int psg_factor_info(psg_ctx* c, double* out) {
+  return guarded([&]() -> int {
+    if (!c || !out) throw Error(PSG_ERR_ARGUMENT, "psg_factor_info: null argument");
+    const FactorState* fs = c->fs;
+    if (!fs) throw Error(PSG_ERR_STATE, "psg_factor_info: no factor");
+    out[0] = fs->solve_mode;
+    out[1] = fs->sf_units;
+    out[2] = fs->n_iso;
+    out[3] = fs->tail_r;
+    out[4] = fs->tail_nchunk;
+    out[5] = fs->nsuper;
     return PSG_OK;
   });
 }

@@ -238,6 +238,45 @@ def test_single_launch_solve_matches_level_scheduled(gpu, monkeypatch, precision
         assert np.linalg.norm(s1 - ref) <= (1e-9 if precision == "fp64" else 5e-7 * cond) * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("m,n,k,frac,tail_min", [(1500, 4500, 5, 1.0, 64), (1500, 4500, 5, 1.0, 700),
+                                                 (3000, 9000, 6, 0.6, 128)])
+def test_dense_tail_inverse_solve(gpu, monkeypatch, precision, m, n, k, frac, tail_min):
+    """The explicit inverse of the dense top of the tree gives the same P^-1 r
+    as substitution (level-scheduled solves) and as LAPACK."""
+    _say(f"dense-tail solve {precision} m={m} core={frac} tail_min={tail_min}")
+    _, lp = _lp(m, n, k, 71)
+    A = lp.A
+    rng = np.random.default_rng(72)
+    cand = np.flatnonzero(np.diff(A.indptr) >= 2)
+    core = np.sort(rng.choice(cand, int(frac * m), replace=False))
+    theta = _theta(n, 73, spread=2.0)
+    monkeypatch.setenv("PSG_TAIL_MIN", str(tail_min))
+    outs, infos = [], []
+    for mode in ("level", "single"):
+        monkeypatch.setenv("PSG_SOLVE", mode)
+        obj = HipSpectralStableColumnPreconditioner(A, protected=~lp.pos, precision=precision)
+        obj._call("psg_set_weights", obj._ctx, theta, 1e-2)
+        plan = SymbolicPlan(A, core)
+        assert obj._call("psg_factor", obj._ctx, plan.handle, 0.0, None) == 0
+        info = np.zeros(6)
+        obj._call("psg_factor_info", obj._ctx, info)
+        infos.append(info)
+        r = np.random.default_rng(74).standard_normal(m)
+        outs.append([obj._apply(r) for _ in range(2)])
+        plan.close()
+        obj.close()
+    tail_r = infos[1][3]
+    assert infos[1][0] == 1 and tail_r >= tail_min and infos[1][4] >= 2
+    (l1, _), (s1, s2) = outs
+    tol = 1e-11 if precision == "fp64" else 1e-4
+    assert np.linalg.norm(s1 - l1) <= tol * np.linalg.norm(l1)
+    assert np.linalg.norm(s2 - s1) <= tol * np.linalg.norm(s1)
+    apply_ref, cond = _dense_precond(A, core, 1.0 / theta, 1e-2)
+    ref = apply_ref(r)
+    assert np.linalg.norm(s1 - ref) <= (1e-9 if precision == "fp64" else 5e-6 * cond) * np.linalg.norm(ref)
+
+
 @pytest.mark.parametrize("nsplit", [2, 3, 7])
 def test_apply_H_column_blocked_csr(gpu, monkeypatch, nsplit):
     _say(f"apply_H split={nsplit}")
