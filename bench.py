@@ -240,7 +240,7 @@ def main():
         "kernel_ms": {"Hv": g64["hv_ms"], "apply_P": g64["apply_ms"], "dot": g64["dot_ms"],
                       "Hv_mixed": None if mixed is None else mixed["hv_loop_ms"],
                       "apply_P_mixed": None if mixed is None else mixed["apply_ms"]},
-        "roofline": {"bound": "hbm", "kernel": "P^-1 r: k_fwd_sf + k_bwd_sf (supernodal triangular solves)",
+        "roofline": {"bound": "hbm", "kernel": "P^-1 r: k_fwd_low + k_fwd_sf + k_tail_lower + k_tail_upper + k_bwd_sf + k_bwd_low (triangular solves)",
                      "achieved": round(achieved_apply, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_apply / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": int(bytes_apply), "ms_per_launch": round(g64["apply_ms"], 4)},

@@ -76,6 +76,9 @@ struct FactorState {
   int sf_units = 0;
   DevBuf<int> su_s, su_r0, su_n, su_range, need, flags, bwd_order, iso_col, iso_doff;
   int n_iso = 0;
+  // bottom levels solved level by level (one wave per supernode)
+  std::vector<int> low_ptr;
+  DevBuf<int> low_nodes;
   // dense tail [tail_F, tail_E) = supernodes [tail_s0, tail_s0 + tail_nchunk)
   int tail_r = 0, tail_F = 0, tail_s0 = 0, tail_nchunk = 0;
   DevBuf<int> tail_chunk_of, tail_lower_tiles, tail_upper_tiles;
@@ -666,6 +669,41 @@ __global__ void __launch_bounds__(kBlock) k_fwd_sf(SolveArgs<T> a) {
   }
 }
 
+// Sum over the 16 lanes of each lane group (lane bits 0-3) of 16 values per
+// lane, transposed: afterwards lane l holds the group sum of acc[l & 15].
+// 8 + 4 + 2 + 1 exchanges instead of 16 separate reductions.
+template <class T>
+__device__ inline T transpose_reduce16(T (&acc)[16], int lane) {
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const bool hi = lane & 8;
+    const T send = hi ? acc[t] : acc[t + 8];
+    const T keep = hi ? acc[t + 8] : acc[t];
+    acc[t] = keep + __shfl_xor(send, 8);
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const bool hi = lane & 4;
+    const T send = hi ? acc[t] : acc[t + 4];
+    const T keep = hi ? acc[t + 4] : acc[t];
+    acc[t] = keep + __shfl_xor(send, 4);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const bool hi = lane & 2;
+    const T send = hi ? acc[t] : acc[t + 2];
+    const T keep = hi ? acc[t + 2] : acc[t];
+    acc[t] = keep + __shfl_xor(send, 2);
+  }
+  {
+    const bool hi = lane & 1;
+    const T send = hi ? acc[0] : acc[1];
+    const T keep = hi ? acc[1] : acc[0];
+    acc[0] = keep + __shfl_xor(send, 1);
+  }
+  return acc[0];
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
   __shared__ T sD[kW][kW + 1];
@@ -702,15 +740,22 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
       row = a.rows[a.rptr[s] + r0 + tid];
       tg[tid] = a.snode_of[row];
     }
-    // panel segment of this lane's column, fetched before waiting on x
+    // panel block of this wave (rows [64 wv, 64 wv + 64) of the unit),
+    // fetched before waiting on x: lane (rq, cg) = (lane & 15, lane >> 4)
+    // holds rows rq + 16k and columns cg + 4j, so every load instruction
+    // reads 16 consecutive rows of 4 columns -- whole cache lines
     const int i0 = wv * kWave;
-    const int cnt_rows = max(0, min(kWave, nrow - i0));
-    T v[kWave];
+    const int rq = lane & 15, cg = lane >> 4;
+    T v[4][16];
     {
-      const T* P = a.panels + a.poff[s] + (int64_t)lane * nr + r0 + i0;
-      const bool col_ok = lane < w && cnt_rows > 0;
+      const T* P = a.panels + a.poff[s] + r0 + i0 + rq;
 #pragma unroll
-      for (int i = 0; i < kWave; ++i) v[i] = (col_ok && i < cnt_rows) ? P[i] : T(0);
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int c = cg + 4 * j;
+          v[k][j] = (c < w && i0 + rq + 16 * k < nrow) ? P[(int64_t)c * nr + 16 * k] : T(0);
+        }
     }
     __syncthreads();
     if (tid < nrow && (tid == 0 || tg[tid - 1] != tg[tid])) wait_at_least(&a.xdone[tg[tid]], 1, a.err, a.sleep_max);
@@ -718,16 +763,18 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
     if (a.trace && tid == 0) a.trace[6 * (int64_t)u + 4] = wall_clock64();
     xr[tid] = (tid < nrow) ? load_agent(a.x + row) : T(0);
     __syncthreads();
-    // partial[c] = sum_rows L[r][c] x[row]: lane = column c, wave wv = rows
-    // [64 wv, 64 wv + 64) -- each lane streams a contiguous column segment
+    // partial[c] = sum over the wave's rows of L[r][c] x[row]
     {
-      T acc0 = T(0), acc1 = T(0);
+      T acc[16];
 #pragma unroll
-      for (int i = 0; i < kWave; i += 2) {
-        acc0 += v[i] * xr[i0 + i];
-        acc1 += v[i + 1] * xr[i0 + i + 1];
+      for (int j = 0; j < 16; ++j) acc[j] = T(0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const T xk = xr[i0 + rq + 16 * k];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc[j] += v[k][j] * xk;
       }
-      part_sh[wv][lane] = acc0 + acc1;
+      part_sh[wv][cg + 4 * rq] = transpose_reduce16(acc, lane);
     }
     __syncthreads();
     if (tid < w && nrow > 0) {
@@ -763,6 +810,102 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
 }
 
 
+
+// ---------------------------------------------------------------------------
+// bottom levels of the tree: one wave per supernode, one launch per level.
+// The launch boundary orders the levels, so no counters are needed; the
+// wide bottom levels (thousands of small supernodes) run at full occupancy
+// instead of one workgroup per unit.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_fwd_low(int count, const int* __restrict__ nodes, const int* __restrict__ sfirst, const int64_t* __restrict__ rptr,
+          const int* __restrict__ rows, const int64_t* __restrict__ poff, const int64_t* __restrict__ doff,
+          const T* __restrict__ panels, const T* __restrict__ dinv, T* __restrict__ b, T* __restrict__ y) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int idx = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (idx >= count) return;  // uniform per wave
+  const int s = nodes[idx];
+  const int f = sfirst[s];
+  const int w = sfirst[s + 1] - f;
+  const int64_t nr = rptr[s + 1] - rptr[s];
+  const T* D = dinv + doff[s];
+  // y_s = L_ss^-1 b_s, lane j owns y_j
+  const T bj = (lane < w) ? b[f + lane] : T(0);
+  T yj = T(0);
+  for (int k = 0; k < w; ++k) {
+    const T bk = __shfl(bj, k);
+    if (lane < w) yj += D[(int64_t)k * w + lane] * bk;
+  }
+  if (lane < w) y[f + lane] = yj;
+  // b[rows below] -= L_below y_s, lane = row
+  const T* P = panels + poff[s];
+  const int* R = rows + rptr[s];
+  for (int64_t i0 = w; i0 < nr; i0 += kWave) {
+    const int64_t i = i0 + lane;
+    T acc0 = T(0), acc1 = T(0);
+    int c = 0;
+    for (; c + 1 < w; c += 2) {
+      const T y0 = __shfl(yj, c), y1 = __shfl(yj, c + 1);
+      if (i < nr) {
+        acc0 += P[(int64_t)c * nr + i] * y0;
+        acc1 += P[(int64_t)(c + 1) * nr + i] * y1;
+      }
+    }
+    if (c < w) {
+      const T y0 = __shfl(yj, c);
+      if (i < nr) acc0 += P[(int64_t)c * nr + i] * y0;
+    }
+    if (i < nr) __hip_atomic_fetch_add(b + R[i], -(acc0 + acc1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_bwd_low(int count, const int* __restrict__ nodes, const int* __restrict__ sfirst, const int64_t* __restrict__ rptr,
+          const int* __restrict__ rows, const int64_t* __restrict__ poff, const int64_t* __restrict__ doff,
+          const T* __restrict__ panels, const T* __restrict__ dinv, const T* __restrict__ y, T* __restrict__ x) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int idx = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (idx >= count) return;  // uniform per wave
+  const int s = nodes[idx];
+  const int f = sfirst[s];
+  const int w = sfirst[s + 1] - f;
+  const int64_t nr = rptr[s + 1] - rptr[s];
+  const T* P = panels + poff[s];
+  const int* R = rows + rptr[s];
+  // acc_c = sum_{below rows} L[i][c] x[row_i]; lane (rq, cg) takes rows
+  // rq + 16k of each 64-row chunk and columns cg + 4j
+  const int rq = lane & 15, cg = lane >> 4;
+  T acc[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = T(0);
+  for (int64_t i0 = w; i0 < nr; i0 += kWave) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t i = i0 + rq + 16 * k;
+      if (i < nr) {
+        const T xk = x[R[i]];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int c = cg + 4 * j;
+          if (c < w) acc[j] += P[(int64_t)c * nr + i] * xk;
+        }
+      }
+    }
+  }
+  const T red = transpose_reduce16(acc, lane);  // column cg + 4 rq
+  const int c_own = cg + 4 * rq;
+  const T t = (c_own < w) ? y[f + c_own] - red : T(0);
+  // x_s = L_ss^-T t: lane j sums D(c, j) t_c over c >= j
+  T xj = T(0);
+  for (int c = 0; c < w; ++c) {
+    const T tc = __shfl(t, (c >> 2) + 16 * (c & 3));
+    if (lane < w && c >= lane) xj += dinv[doff[s] + (int64_t)lane * w + c] * tc;
+  }
+  if (lane < w) x[f + lane] = xj;
+}
+
 // ---------------------------------------------------------------------------
 // dense tail: the top of the tree is a dense lower-triangular block L_T
 // (columns [F, E), every later column in every structure).  Its explicit
@@ -772,23 +915,23 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
 constexpr int kGT = 64;   // GEMM tile edge
 constexpr int kGK = 32;   // GEMM k-chunk
 
-// Ld (r x r, column-major) <- lower part of the tail panels, zeros above
+// Ld (r x r, column-major, zeroed beforehand) <- the tail panels.  One block
+// per (tail supernode, column): the panel column is contiguous, its row
+// indices place it in Ld.
 template <class T>
-__global__ void k_tail_gather(int r, int F, const int* __restrict__ chunk_of, const int* __restrict__ sfirst,
-                              const int64_t* __restrict__ rptr, const int64_t* __restrict__ poff,
-                              const T* __restrict__ panels, T* __restrict__ Ld) {
-  const int64_t total = (int64_t)r * r;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(e / r), i = (int)(e % r);
-    T v = T(0);
-    if (i >= c) {
-      const int s = chunk_of[c];
-      const int cf = sfirst[s] - F;
-      const int64_t nr = rptr[s + 1] - rptr[s];
-      v = panels[poff[s] + (int64_t)(c - cf) * nr + (i - cf)];
-    }
-    Ld[e] = v;
-  }
+__global__ void k_tail_scatter(int r, int F, int s0, const int* __restrict__ sfirst,
+                               const int64_t* __restrict__ rptr, const int* __restrict__ rows,
+                               const int64_t* __restrict__ poff, const int* __restrict__ col_snode,
+                               const T* __restrict__ panels, T* __restrict__ Ld) {
+  const int col = blockIdx.x;  // tail-local column
+  const int s = col_snode[col];
+  const int f = sfirst[s];
+  const int c = F + col - f;   // column within the supernode
+  const int64_t nr = rptr[s + 1] - rptr[s];
+  const int* R = rows + rptr[s];
+  const T* Pc = panels + poff[s] + (int64_t)c * nr;
+  T* Lc = Ld + (int64_t)col * r - F;
+  for (int64_t k = c + threadIdx.x; k < nr; k += blockDim.x) Lc[R[k]] = Pc[k];
 }
 
 // X diagonal blocks <- stored block inverses
@@ -1030,8 +1173,9 @@ static void tail_invert(psg_ctx* c, const T* panels, const T* dinv, T* Ld, T* X,
   hipStream_t st = c->stream;
   const int r = fs->tail_r;
   const int64_t rr = (int64_t)r * r;
-  hipLaunchKernelGGL((k_tail_gather<T>), dim3(grid_for(rr, kBlock, 65535)), dim3(kBlock), 0, st, r, fs->tail_F,
-                     fs->tail_chunk_of.p, fs->sfirst.p, fs->rptr.p, fs->poff.p, panels, Ld);
+  HIP_CHECK(hipMemsetAsync(Ld, 0, rr * sizeof(T), st));
+  hipLaunchKernelGGL((k_tail_scatter<T>), dim3(r), dim3(kBlock), 0, st, r, fs->tail_F, fs->tail_s0, fs->sfirst.p,
+                     fs->rptr.p, fs->rows.p, fs->poff.p, fs->tail_chunk_of.p, panels, Ld);
   HIP_LAUNCH_CHECK();
   HIP_CHECK(hipMemsetAsync(X, 0, rr * sizeof(T), st));
   hipLaunchKernelGGL((k_tail_diag<T>), dim3(fs->tail_nchunk), dim3(kBlock), 0, st, r, fs->tail_F, fs->tail_nchunk,
@@ -1140,31 +1284,41 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
       if (P.sparent[sn] >= 0) level[P.sparent[sn]] = std::max(level[P.sparent[sn]], level[sn] + 1);
     for (int sn = P.nsuper - 1; sn >= 0; --sn)
       if (P.sparent[sn] >= 0) depth[sn] = depth[P.sparent[sn]] + 1;
-    // dense tail: the longest chain ending at a root whose structures are all
-    // "every later column" -- inverted explicitly when large enough
+    // dense tail: a postorder suffix [s0, q] of the subtree of a root q is
+    // closed under "parent of", so its columns [F, E) form the trailing
+    // diagonal block of L with nothing else below it.  The longest suffix
+    // whose block holds at least a fraction PSG_TAIL_DENSITY of nonzeros is
+    // inverted explicitly when it has PSG_TAIL_MIN columns or more.
     {
-      const char* tm = getenv("PSG_TAIL_MIN");
-      const int tail_min = tm ? atoi(tm) : 2048;
+      auto knobd = [](const char* name, double dflt) {
+        const char* v = getenv(name);
+        return v ? atof(v) : dflt;
+      };
+      const int tail_min = (int)knobd("PSG_TAIL_MIN", 2048);
+      const int tail_max = (int)knobd("PSG_TAIL_MAX", 24576);
+      const double rho = knobd("PSG_TAIL_DENSITY", 0.8);
+      std::vector<int> sub(P.nsuper, 1);  // subtree sizes (postorder: children first)
+      for (int sn = 0; sn < P.nsuper; ++sn)
+        if (P.sparent[sn] >= 0) sub[P.sparent[sn]] += sub[sn];
       int best_q = -1, best_s0 = -1, best_r = 0;
       for (int q = 0; q < P.nsuper; ++q) {
         if (P.sparent[q] >= 0) continue;
         const int E = P.sfirst[q + 1];
-        int s0 = q;
-        while (true) {
-          const int sn = s0;
-          const int64_t nr = P.rptr[sn + 1] - P.rptr[sn];
-          if (nr != E - P.sfirst[sn]) break;
-          if (sn != q && P.sparent[sn] != sn + 1) break;
-          if (sn == 0) { s0 = -1; break; }
-          s0 = sn - 1;
+        double nz = 0.0;
+        for (int sn = q; sn > q - sub[q]; --sn) {
+          const double w = P.sfirst[sn + 1] - P.sfirst[sn];
+          const double nr = (double)(P.rptr[sn + 1] - P.rptr[sn]);
+          nz += w * nr - w * (w - 1) / 2;
+          const double r = E - P.sfirst[sn];
+          if (r > tail_max) break;
+          const double dens = nz / (r * (r + 1) / 2);
+          if (dens >= rho && r > best_r) { best_r = (int)r; best_q = q; best_s0 = sn; }
+          if (dens < 0.5 * rho) break;
         }
-        const int first = s0 + 1;
-        const int r = E - P.sfirst[first];
-        if (first <= q && r > best_r) { best_r = r; best_q = q; best_s0 = first; }
       }
       fs->tail_r = 0;
       fs->tail_nchunk = 0;
-      if (best_q >= 0 && best_r >= tail_min) {
+      if (best_q >= 0 && best_r >= tail_min && best_q > best_s0) {
         fs->tail_r = best_r;
         fs->tail_s0 = best_s0;
         fs->tail_nchunk = best_q - best_s0 + 1;
@@ -1172,6 +1326,11 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
       }
     }
     auto in_tail = [&](int sn) { return fs->tail_nchunk && sn >= fs->tail_s0 && sn < fs->tail_s0 + fs->tail_nchunk; };
+    // backward depth counts from the tail: the tail is solved before the
+    // backward kernel starts, so its children begin the dependency chains
+    if (fs->tail_nchunk)
+      for (int sn = P.nsuper - 1; sn >= 0; --sn)
+        depth[sn] = (P.sparent[sn] < 0 || in_tail(P.sparent[sn])) ? 0 : depth[P.sparent[sn]] + 1;
     // isolated 1x1 supernodes (rows no active column touches) are solved
     // elementwise, x = b / l_kk^2, outside the dependency-driven kernels
     std::vector<int> iso_col, iso_doff;
@@ -1188,6 +1347,30 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
       } else if (!in_tail(sn)) {
         order.push_back(sn);
       }
+    }
+    // bottom levels with many supernodes go to the level kernels; the rest
+    // of the tree (above them, below the tail) to the single-launch kernels
+    {
+      const char* v1 = getenv("PSG_SF_LOW_MIN");
+      const char* v2 = getenv("PSG_SF_LOW_MAX");
+      const int low_min = v1 ? atoi(v1) : 512;
+      const int low_max = v2 ? atoi(v2) : 64;
+      int maxlev = 0;
+      for (int sn : order) maxlev = std::max(maxlev, level[sn]);
+      std::vector<int> per(maxlev + 2, 0);
+      for (int sn : order) per[level[sn]]++;
+      int K = 0;
+      while (K < low_max && K <= maxlev && per[K] >= low_min) ++K;
+      fs->low_ptr.assign(K + 1, 0);
+      for (int L = 0; L < K; ++L) fs->low_ptr[L + 1] = fs->low_ptr[L] + per[L];
+      std::vector<int> low(fs->low_ptr[K]), fill(fs->low_ptr.begin(), fs->low_ptr.end());
+      std::vector<int> rest;
+      for (int sn : order) {
+        if (level[sn] < K) low[fill[level[sn]]++] = sn;
+        else rest.push_back(sn);
+      }
+      fs->low_nodes.upload(low, s);
+      order.swap(rest);
     }
     // ties: the longer remaining dependency chain first
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -1470,6 +1653,15 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
                        fs->iso_col.p, fs->iso_doff.p, dinv, vb, vy, vx);
     HIP_LAUNCH_CHECK();
   }
+  const int nlow = (int)fs->low_ptr.size() - 1;
+  const int wpb = kBlock / kWave;
+  for (int L = 0; L < nlow; ++L) {
+    const int cnt_l = fs->low_ptr[L + 1] - fs->low_ptr[L];
+    hipLaunchKernelGGL((k_fwd_low<T>), dim3((cnt_l + wpb - 1) / wpb), dim3(kBlock), 0, s, cnt_l,
+                       fs->low_nodes.p + fs->low_ptr[L], fs->sfirst.p, fs->rptr.p, fs->rows.p, fs->poff.p,
+                       fs->doff.p, panels, dinv, vb, vy);
+    HIP_LAUNCH_CHECK();
+  }
   if (fs->sf_units) {
     hipLaunchKernelGGL((k_fwd_sf<T>), dim3(grid), dim3(kBlock), 0, s, a);
     HIP_LAUNCH_CHECK();
@@ -1482,6 +1674,13 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   }
   if (fs->sf_units) {
     hipLaunchKernelGGL((k_bwd_sf<T>), dim3(grid), dim3(kBlock), 0, s, a);
+    HIP_LAUNCH_CHECK();
+  }
+  for (int L = nlow - 1; L >= 0; --L) {
+    const int cnt_l = fs->low_ptr[L + 1] - fs->low_ptr[L];
+    hipLaunchKernelGGL((k_bwd_low<T>), dim3((cnt_l + wpb - 1) / wpb), dim3(kBlock), 0, s, cnt_l,
+                       fs->low_nodes.p + fs->low_ptr[L], fs->sfirst.p, fs->rptr.p, fs->rows.p, fs->poff.p,
+                       fs->doff.p, panels, dinv, vy, vx);
     HIP_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL((k_perm_out<T>), dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, fs->perm.p,

@@ -415,7 +415,10 @@ class HipSpectralStableColumnPreconditioner:
             plan.close()
             scores = getattr(self, "_last_scores", None)
             local = np.searchsorted(self.candidates, self.core)
-            keep = max(1, int(0.85 * self.core.size))
+            # fill grows roughly with the square of the core size on random
+            # patterns: jump close to the fitting size, then refine by 15%
+            ratio = float(np.sqrt(budget / max(plan.info["nnz_L"], 1.0)))
+            keep = max(1, int(min(0.85, max(0.2, 0.95 * ratio)) * self.core.size))
             optional = ~np.isin(self.core, self.mandatory)
             rank = np.full(self.core.size, np.inf)
             if scores is not None and scores.size == self.candidates.size:
