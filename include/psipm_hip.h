@@ -146,6 +146,13 @@ int psg_time_kernels(psg_ctx* ctx, int32_t reps, double* out);
  * environment variable PSG_SF_TRACE set; *nunits = 0 otherwise.  No
  * counterpart in the reference. */
 int psg_solve_trace(psg_ctx* ctx, int64_t* out, int64_t cap, int64_t* nunits);
+/* Dense kernels of the factor's tail on their own: a (n x n, column-major,
+ * lower part read) is replaced by X = L^-1 with L L' = a, computed on the GPU
+ * in the context's precision by the blocked MFMA Cholesky and the recursive
+ * inversion the factor uses for its dense tail.  Returns 0, or 1 + the
+ * column of a non-positive pivot.  No counterpart in the reference (CHOLMOD
+ * does the whole factor there, PS:855-885); used to test those kernels. */
+int psg_dense_cholinv(psg_ctx* ctx, int64_t n, double* a);
 /* Diagnostics of the factor's solve layout, out[6] = {solve mode (0 level
  * scheduled, 1 single launch), dependency-driven units, isolated 1x1
  * supernodes, dense-tail order r (0: none), dense-tail supernodes, nsuper}.

@@ -43,13 +43,18 @@ struct LevelRange {
   int tile_begin, tile_end;   // update tiles
 };
 
-// one product C = alpha A B of the tail inversion (column-major blocks)
-struct GemmDesc {
+// one product C = beta C + alpha A op(B) of the dense tail factorisation
+// (column-major blocks of the tail buffer or of the scratch buffer)
+struct MGemm {
   int M, N, K;
-  int64_t a_off, b_off, c_off;  // element offsets into the A, B, C base buffers
+  int64_t a_off, b_off, c_off;
   int lda, ldb, ldc;
-  int a_sel, b_sel, c_sel;      // 0: Ld, 1: X, 2: scratch
-  double alpha;
+  int a_sel, b_sel, c_sel;  // 0: tail buffer, 1: scratch
+  int transB;               // op(B) = B' with B stored N x K
+  int lower;                // write only entries with row - col + diag >= 0
+  int diag;
+  int tri;                  // 1: B lower triangular (k >= col), 2: A lower triangular (k <= row)
+  double alpha, beta;
 };
 
 struct FactorState {
@@ -79,16 +84,20 @@ struct FactorState {
   // bottom levels solved level by level (one wave per supernode)
   std::vector<int> low_ptr;
   DevBuf<int> low_nodes;
-  // dense tail [tail_F, tail_E) = supernodes [tail_s0, tail_s0 + tail_nchunk)
+  // dense tail [tail_F, tail_F + tail_r) = supernodes [tail_s0, tail_s0 + tail_nchunk),
+  // factored and inverted as one dense block; its inverse X = L_T^-1
+  // overwrites L_T in tail_L
   int tail_r = 0, tail_F = 0, tail_s0 = 0, tail_nchunk = 0;
   DevBuf<int> tail_chunk_of, tail_lower_tiles, tail_upper_tiles;
   int tail_nlower = 0, tail_nupper = 0;
-  DevBuf<double> tail_Ld64, tail_X64, tail_S64;
-  DevBuf<float> tail_Ld32, tail_X32, tail_S32;
-  struct GemmLevel { int desc_begin, ndesc, tile_begin, ntile; };
-  std::vector<GemmLevel> gemm_levels;  // two launches per level (T = B Ai, then X21 = -Ci T)
-  DevBuf<GemmDesc> gemm_descs;
-  DevBuf<int> gemm_tile_desc, gemm_tile_mn;
+  DevBuf<double> tail_L64, tail_S64;
+  DevBuf<float> tail_L32, tail_S32;
+  struct TailStep { int kind, j, w; MGemm g; };  // kind: 0 potrf, 1 trsm, 2 update
+  std::vector<TailStep> tail_steps;
+  struct InvLaunch { int tile_begin, ntile; };
+  std::vector<InvLaunch> inv_launches;  // two per level of the recursive inversion
+  DevBuf<MGemm> inv_descs;
+  DevBuf<int> inv_tile_desc, inv_tile_mn;
   DevBuf<long long> trace;  // allocated when PSG_SF_TRACE is set
   DevBuf<double> sf_part64;
   DevBuf<float> sf_part32;
@@ -142,24 +151,11 @@ __global__ void k_assemble_diag(int m, const int* __restrict__ perm, const int64
 // POTRF of every diagonal block of a level in LDS; writes L_ss back to the
 // panel and L_ss^-1 (column-major w x w) to dinv.  A non-positive pivot
 // records its permuted column in *fail (atomic min).
+// Cholesky of the w x w block held in LDS (lower part, in place); true when
+// a pivot is not positive and finite.  Called by all threads of the block.
 template <class T>
-__global__ void __launch_bounds__(kBlock)
-k_potrf(const int* __restrict__ nodes, const int* __restrict__ sfirst, const int64_t* __restrict__ rptr,
-        const int64_t* __restrict__ poff, const int64_t* __restrict__ doff, T* __restrict__ panels,
-        T* __restrict__ dinv, int* __restrict__ fail) {
-  __shared__ T sA[kW][kW + 1];
-  __shared__ T tmp[kW];
-  const int s = nodes[blockIdx.x];
-  const int f = sfirst[s];
-  const int w = sfirst[s + 1] - f;
-  const int64_t nr = rptr[s + 1] - rptr[s];
-  T* P = panels + poff[s];
+__device__ inline bool lds_potrf(T (*sA)[kW + 1], int w) {
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < w * w; idx += blockDim.x) {
-    const int c = idx / w, r = idx % w;
-    sA[r][c] = (r >= c) ? P[(int64_t)c * nr + r] : T(0);
-  }
-  __syncthreads();
   const int row = tid & (kW - 1);
   const int grp = tid / kW;  // 0..3
   bool bad = false;
@@ -178,15 +174,15 @@ k_potrf(const int* __restrict__ nodes, const int* __restrict__ sfirst, const int
       for (int k = j + 1 + grp; k <= row; k += kBlock / kW) sA[row][k] -= lij * sA[k][j];
     }
   }
-  if (bad && tid == 0) atomicMin(fail, f);
   __syncthreads();
-  for (int idx = tid; idx < w * w; idx += blockDim.x) {
-    const int c = idx / w, r = idx % w;
-    if (r >= c) P[(int64_t)c * nr + r] = sA[r][c];
-  }
-  __syncthreads();
-  // in-place inverse of the lower-triangular block, last column first:
-  // inv[j][j] = 1/l_jj, inv[j+1:, j] = -inv[j+1:, j+1:] l[j+1:, j] / l_jj
+  return bad;
+}
+
+// In-place inverse of the lower-triangular w x w block in LDS, last column
+// first: inv[j][j] = 1/l_jj, inv[j+1:, j] = -inv[j+1:, j+1:] l[j+1:, j] / l_jj.
+template <class T>
+__device__ inline void lds_trinv(T (*sA)[kW + 1], T* tmp, int w) {
+  const int tid = threadIdx.x;
   for (int j = w - 1; j >= 0; --j) {
     if (tid > j && tid < w) {
       T acc = T(0);
@@ -198,6 +194,35 @@ k_potrf(const int* __restrict__ nodes, const int* __restrict__ sfirst, const int
     if (tid > j && tid < w) sA[tid][j] = -tmp[tid] * sA[j][j];
     __syncthreads();
   }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_potrf(const int* __restrict__ nodes, const int* __restrict__ sfirst, const int64_t* __restrict__ rptr,
+        const int64_t* __restrict__ poff, const int64_t* __restrict__ doff, T* __restrict__ panels,
+        T* __restrict__ dinv, int* __restrict__ fail) {
+  __shared__ T sA[kW][kW + 1];
+  __shared__ T tmp[kW];
+  const int s = nodes[blockIdx.x];
+  const int f = sfirst[s];
+  const int w = sfirst[s + 1] - f;
+  const int64_t nr = rptr[s + 1] - rptr[s];
+  T* P = panels + poff[s];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < w * w; idx += blockDim.x) {
+    const int c = idx / w, r = idx % w;
+    sA[r][c] = (r >= c) ? P[(int64_t)c * nr + r] : T(0);
+  }
+  __syncthreads();
+  const bool bad = lds_potrf(sA, w);
+  if (bad && tid == 0) atomicMin(fail, f);
+  __syncthreads();
+  for (int idx = tid; idx < w * w; idx += blockDim.x) {
+    const int c = idx / w, r = idx % w;
+    if (r >= c) P[(int64_t)c * nr + r] = sA[r][c];
+  }
+  __syncthreads();
+  lds_trinv(sA, tmp, w);
   T* D = dinv + doff[s];
   for (int idx = tid; idx < w * w; idx += blockDim.x) {
     const int c = idx / w, r = idx % w;
@@ -912,8 +937,6 @@ k_bwd_low(int count, const int* __restrict__ nodes, const int* __restrict__ sfir
 // inverse X = L_T^-1 turns the tail of both triangular solves -- a chain of
 // ~r/64 dependent steps -- into one bandwidth-bound product each.
 // ---------------------------------------------------------------------------
-constexpr int kGT = 64;   // GEMM tile edge
-constexpr int kGK = 32;   // GEMM k-chunk
 
 // Ld (r x r, column-major, zeroed beforehand) <- the tail panels.  One block
 // per (tail supernode, column): the panel column is contiguous, its row
@@ -934,79 +957,189 @@ __global__ void k_tail_scatter(int r, int F, int s0, const int* __restrict__ sfi
   for (int64_t k = c + threadIdx.x; k < nr; k += blockDim.x) Lc[R[k]] = Pc[k];
 }
 
-// X diagonal blocks <- stored block inverses
+// C = beta C + alpha A op(B) on MFMA, one 128 x 128 output tile per
+// workgroup, four waves of 64 x 64 (4 x 4 blocks of 16 x 16 x 4 MFMAs).
+// The MFMA takes op(B) as its first operand and A as its second, so that each
+// lane's four results lie in four columns of one row of C and a store
+// instruction writes 16 consecutive rows of C.  Either one product
+// (descs == nullptr; tiles from blockIdx over ntm row tiles) or a batch given
+// by per-tile descriptor indices.
+template <class T> struct MfmaOp;
+// Both instructions read A(i, k) from lane i + 16 k and B(k, j) from lane
+// j + 16 k; they differ in where result v of a lane lands: row 4 v + lane/16
+// for the FP64 instruction, row 4 (lane/16) + v for the FP32 one (column
+// lane % 16 in both).
+template <> struct MfmaOp<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  __device__ static inline v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int v) { return 4 * v + (lane >> 4); }
+};
+template <> struct MfmaOp<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  __device__ static inline v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+constexpr int kMT = 128;  // output tile edge
+constexpr int kKT = 16;   // k-chunk staged in LDS
+
 template <class T>
-__global__ void k_tail_diag(int r, int F, int nchunk, int s0, const int* __restrict__ sfirst,
-                            const int64_t* __restrict__ doff, const T* __restrict__ dinv, T* __restrict__ X) {
-  const int k = blockIdx.x;
-  if (k >= nchunk) return;
-  const int s = s0 + k;
-  const int cf = sfirst[s] - F;
-  const int w = sfirst[s + 1] - sfirst[s];
-  const T* D = dinv + doff[s];
-  for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
-    const int c = idx / w, a = idx % w;
-    X[(int64_t)(cf + c) * r + (cf + a)] = D[idx];
+__global__ void __launch_bounds__(kBlock)
+k_mfma_gemm(MGemm single, int ntm, const MGemm* __restrict__ descs, const int* __restrict__ tile_desc,
+            const int* __restrict__ tile_mn, T* __restrict__ base0, T* __restrict__ base1) {
+  using V = typename MfmaOp<T>::v4;
+  __shared__ T sA[kKT][kMT + 4];
+  __shared__ T sB[kKT][kMT + 4];
+  MGemm d;
+  int tm, tn;
+  if (descs) {
+    d = descs[tile_desc[blockIdx.x]];
+    tm = tile_mn[2 * blockIdx.x];
+    tn = tile_mn[2 * blockIdx.x + 1];
+  } else {
+    d = single;
+    tm = blockIdx.x % ntm;
+    tn = blockIdx.x / ntm;
+  }
+  const int i0 = tm * kMT, j0 = tn * kMT;
+  if (d.lower && (i0 + kMT - 1) - j0 + d.diag < 0) return;  // tile above the diagonal
+  const T* A = (d.a_sel ? base1 : base0) + d.a_off;
+  const T* B = (d.b_sel ? base1 : base0) + d.b_off;
+  T* C = (d.c_sel ? base1 : base0) + d.c_off;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const int wm = wv & 1, wn = wv >> 1;
+  V acc[4][4];
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[ti][tj][v] = T(0);
+  T ra[8], rb[8];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int idx = tid + kBlock * e;
+      {  // A(i, k), consecutive lanes walk a column
+        const int gi = i0 + (idx & (kMT - 1)), gk = k0 + (idx >> 7);
+        ra[e] = (gi < d.M && gk < d.K) ? A[(int64_t)gk * d.lda + gi] : T(0);
+      }
+      if (d.transB) {  // op(B)(k, j) = B(j, k): consecutive lanes walk a column of B
+        const int gj = j0 + (idx & (kMT - 1)), gk = k0 + (idx >> 7);
+        rb[e] = (gj < d.N && gk < d.K) ? B[(int64_t)gk * d.ldb + gj] : T(0);
+      } else {  // B(k, j): consecutive lanes walk k down a column
+        const int gk = k0 + (idx & (kKT - 1)), gj = j0 + (idx >> 4);
+        rb[e] = (gj < d.N && gk < d.K) ? B[(int64_t)gj * d.ldb + gk] : T(0);
+      }
+    }
+  };
+  // a lower-triangular operand bounds the k-range of this tile
+  int kbeg = 0, kend = d.K;
+  if (d.tri == 1) kbeg = (j0 / kKT) * kKT;
+  if (d.tri == 2) kend = min(d.K, i0 + kMT);
+  load(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += kKT) {
+    __syncthreads();  // the previous chunk's LDS reads are done
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int idx = tid + kBlock * e;
+      sA[idx >> 7][idx & (kMT - 1)] = ra[e];
+      if (d.transB) sB[idx >> 7][idx & (kMT - 1)] = rb[e];
+      else sB[idx & (kKT - 1)][idx >> 4] = rb[e];
+    }
+    __syncthreads();
+    if (k0 + kKT < kend) load(k0 + kKT);  // next chunk in flight during the MFMAs
+#pragma unroll
+    for (int kk = 0; kk < kKT; kk += 4) {
+      T fa[4], fb[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        fa[t] = sA[kk + (lane >> 4)][wm * 64 + t * 16 + (lane & 15)];
+        fb[t] = sB[kk + (lane >> 4)][wn * 64 + t * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) acc[ti][tj] = MfmaOp<T>::mma(fb[tj], fa[ti], acc[ti][tj]);
+    }
+  }
+  // acc[ti][tj][v] = C(rowbase + (lane & 15), colbase + row(lane, v)): the
+  // result's rows are columns of C (operands swapped)
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gi = i0 + wm * 64 + ti * 16 + (lane & 15);
+        const int gj = j0 + wn * 64 + tj * 16 + MfmaOp<T>::row(lane, v);
+        if (gi < d.M && gj < d.N && (!d.lower || gi - gj + d.diag >= 0)) {
+          T* pc = C + (int64_t)gj * d.ldc + gi;
+          T val = (T)(d.alpha * (double)acc[ti][tj][v]);
+          if (d.beta != 0.0) val += (T)d.beta * *pc;
+          *pc = val;
+        }
+      }
+}
+
+// Dense tail, 64-column step j: factor the w x w diagonal block in LDS and
+// leave its inverse in place (the factor itself is not needed afterwards:
+// the panel solve and the inversion use the inverse).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_dense_potrf(T* __restrict__ L, int r, int j, int w, int F, int* __restrict__ fail) {
+  __shared__ T sA[kW][kW + 1];
+  __shared__ T tmp[kW];
+  T* Pj = L + (int64_t)j * r + j;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < w * w; idx += blockDim.x) {
+    const int c = idx / w, rr = idx % w;
+    sA[rr][c] = (rr >= c) ? Pj[(int64_t)c * r + rr] : T(0);
+  }
+  __syncthreads();
+  const bool bad = lds_potrf(sA, w);
+  if (bad && tid == 0) atomicMin(fail, F + j);
+  __syncthreads();
+  lds_trinv(sA, tmp, w);
+  __syncthreads();
+  for (int idx = tid; idx < w * w; idx += blockDim.x) {
+    const int c = idx / w, rr = idx % w;
+    if (rr >= c) Pj[(int64_t)c * r + rr] = sA[rr][c];
   }
 }
 
-
-// Batched C = alpha A B (column-major, beta = 0) over 64x64 output tiles.
+// Dense tail, rows below step j: L[i, j:j+w] <- L[i, j:j+w] L_jj^-T, one
+// row per lane, in registers (entry c needs the original entries 0..c, so
+// the entries are replaced from the last down).
 template <class T>
 __global__ void __launch_bounds__(kBlock)
-k_gemm_batched(const GemmDesc* __restrict__ descs, const int* __restrict__ tile_desc,
-               const int* __restrict__ tile_mn, T* __restrict__ Ld, T* __restrict__ X, T* __restrict__ Sc) {
-  __shared__ T sA[kGK][kGT + 1];
-  __shared__ T sB[kGK][kGT + 1];
-  const GemmDesc d = descs[tile_desc[blockIdx.x]];
-  const int tm = tile_mn[2 * blockIdx.x], tn = tile_mn[2 * blockIdx.x + 1];
-  T* bases[3] = {Ld, X, Sc};
-  const T* A = bases[d.a_sel] + d.a_off;
-  const T* B = bases[d.b_sel] + d.b_off;
-  T* C = bases[d.c_sel] + d.c_off;
-  const int tid = threadIdx.x;
-  const int tx = tid % 16, ty = tid / 16;
-  const int i0 = tm * kGT, j0 = tn * kGT;
-  T acc[4][4];
+k_dense_trsm(T* __restrict__ L, int r, int j, int w) {
+  __shared__ T sD[kW][kW + 1];
+  const T* D = L + (int64_t)j * r + j;
+  for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
+    const int k = idx / w, c = idx % w;
+    sD[c][k] = (c >= k) ? D[(int64_t)k * r + c] : T(0);
+  }
+  __syncthreads();
+  const int64_t i = (int64_t)j + w + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= r) return;
+  T a[kW];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int k = 0; k < kW; ++k) a[k] = (k < w) ? L[(int64_t)(j + k) * r + i] : T(0);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
-  for (int k0 = 0; k0 < d.K; k0 += kGK) {
-    __syncthreads();
-    for (int idx = tid; idx < kGK * kGT; idx += blockDim.x) {
-      {  // A(gi, gk): consecutive lanes walk a column of A
-        const int kk = idx / kGT, a = idx % kGT;
-        const int gi = i0 + a, gk = k0 + kk;
-        sA[kk][a] = (gi < d.M && gk < d.K) ? A[(int64_t)gk * d.lda + gi] : T(0);
-      }
-      {  // B(gk, gj): consecutive lanes walk a column of B
-        const int a = idx / kGK, kk = idx % kGK;
-        const int gj = j0 + a, gk = k0 + kk;
-        sB[kk][a] = (gj < d.N && gk < d.K) ? B[(int64_t)gj * d.ldb + gk] : T(0);
-      }
-    }
-    __syncthreads();
-    const int kc = min(kGK, d.K - k0);
-    for (int kk = 0; kk < kc; ++kk) {
-      T a[4], b[4];
+  for (int c = kW - 1; c >= 0; --c) {
+    if (c < w) {
+      T acc = T(0);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = sA[kk][ty * 4 + i];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = sB[kk][tx * 4 + j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+      for (int k = 0; k <= c; ++k) acc += a[k] * sD[c][k];
+      a[c] = acc;
+      L[(int64_t)(j + c) * r + i] = acc;
     }
   }
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int gi = i0 + ty * 4 + i, gj = j0 + tx * 4 + j;
-      if (gi < d.M && gj < d.N) C[(int64_t)gj * d.ldc + gi] = (T)(d.alpha * (double)acc[i][j]);
-    }
 }
 
 // y_T += X b_T over tiles (256 rows x 512 columns, lower part only)
@@ -1072,19 +1205,18 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int v) {
 // host orchestration
 // ---------------------------------------------------------------------------
 
-// Tail buffers, GEMV tile lists and the recursive inversion schedule.
+static void setup_dense_schedule(FactorState* fs, int r, int precision, hipStream_t st);
+
+// Tail buffers and GEMV tile lists, then the dense schedule.
 static void setup_tail(psg_ctx* c, const Plan& P) {
   FactorState* fs = c->fs;
   hipStream_t st = c->stream;
   const int r = fs->tail_r, F = fs->tail_F;
   std::vector<int> chunk_of(r);
-  std::vector<int> cbeg(fs->tail_nchunk + 1);
   for (int k = 0; k < fs->tail_nchunk; ++k) {
     const int sn = fs->tail_s0 + k;
-    cbeg[k] = P.sfirst[sn] - F;
     for (int col = P.sfirst[sn]; col < P.sfirst[sn + 1]; ++col) chunk_of[col - F] = sn;
   }
-  cbeg[fs->tail_nchunk] = r;
   fs->tail_chunk_of.upload(chunk_of, st);
   std::vector<int> lt, ut;
   for (int I = 0; I * kTR < r; ++I)
@@ -1095,20 +1227,51 @@ static void setup_tail(psg_ctx* c, const Plan& P) {
   fs->tail_nupper = (int)ut.size() / 2;
   fs->tail_lower_tiles.upload(lt, st);
   fs->tail_upper_tiles.upload(ut, st);
-  // recursive 2x2 inversion: pair neighbouring segments level by level;
-  // for pair (cols [a,b) | [b,c)):  S = L[b:c, a:b] X[a:b, a:b]  (scratch),
-  //                                 X[b:c, a:b] = -X[b:c, b:c] S
+  setup_dense_schedule(fs, r, c->precision, st);
+}
+
+// Blocked Cholesky and inversion schedule of an r x r dense block, and its
+// buffers.
+static void setup_dense_schedule(FactorState* fs, int r, int precision, hipStream_t st) {
+  // right-looking Cholesky: 64-column steps (diagonal factor + inverse, panel
+  // solve, update of the rest of the 256-column block), and after each
+  // 256-column block the update of the trailing matrix
+  constexpr int kOuter = 256;
+  auto upd = [&](int row0, int col0, int M, int N, int kcol, int K) {
+    MGemm g{};
+    g.M = M; g.N = N; g.K = K;
+    g.a_off = (int64_t)kcol * r + row0; g.lda = r;  // L[row0:row0+M, kcol:kcol+K]
+    g.b_off = (int64_t)kcol * r + col0; g.ldb = r;  // L[col0:col0+N, kcol:kcol+K], used transposed
+    g.c_off = (int64_t)col0 * r + row0; g.ldc = r;
+    g.transB = 1; g.lower = 1; g.diag = row0 - col0;
+    g.alpha = -1.0; g.beta = 1.0;
+    return g;
+  };
+  fs->tail_steps.clear();
+  for (int j0 = 0; j0 < r; j0 += kOuter) {
+    const int je = std::min(r, j0 + kOuter);
+    for (int j = j0; j < je; j += kW) {
+      const int w = std::min(kW, je - j);
+      fs->tail_steps.push_back({0, j, w, MGemm{}});
+      if (j + w < r) fs->tail_steps.push_back({1, j, w, MGemm{}});
+      if (j + w < je) fs->tail_steps.push_back({2, j, w, upd(j + w, j + w, r - j - w, je - j - w, j, w)});
+    }
+    if (je < r) fs->tail_steps.push_back({2, j0, je - j0, upd(je, je, r - je, r - je, j0, je - j0)});
+  }
+  // recursive inversion over the 64-column blocks: for neighbouring segments
+  // [a,b) | [b,c):  S = L[b:c, a:b] X[a:b, a:b]  (scratch),
+  //                 X[b:c, a:b] = -X[b:c, b:c] S  (over L[b:c, a:b])
   std::vector<std::pair<int, int>> segs;
-  for (int k = 0; k < fs->tail_nchunk; ++k) segs.emplace_back(cbeg[k], cbeg[k + 1]);
-  std::vector<GemmDesc> descs;
+  for (int j = 0; j < r; j += kW) segs.emplace_back(j, std::min(r, j + kW));
+  std::vector<MGemm> descs;
   std::vector<int> tdesc, tmn;
-  fs->gemm_levels.clear();
-  int64_t scratch_need = 0;
-  auto add_gemm = [&](const GemmDesc& d) {
+  fs->inv_launches.clear();
+  int64_t scratch_need = 1;
+  auto add_gemm = [&](const MGemm& d) {
     const int id = (int)descs.size();
     descs.push_back(d);
-    for (int tm = 0; tm * kGT < d.M; ++tm)
-      for (int tn = 0; tn * kGT < d.N; ++tn) {
+    for (int tm = 0; tm * kMT < d.M; ++tm)
+      for (int tn = 0; tn * kMT < d.N; ++tn) {
         tdesc.push_back(id);
         tmn.push_back(tm);
         tmn.push_back(tn);
@@ -1116,75 +1279,101 @@ static void setup_tail(psg_ctx* c, const Plan& P) {
   };
   while (segs.size() > 1) {
     std::vector<std::pair<int, int>> next;
-    FactorState::GemmLevel g1{(int)descs.size(), 0, (int)tdesc.size(), 0};
-    int64_t soff = 0;
     std::vector<std::tuple<int, int, int, int64_t>> pairs;
+    FactorState::InvLaunch g1{(int)tdesc.size(), 0};
+    int64_t soff = 0;
     for (size_t q = 0; q + 1 < segs.size(); q += 2) {
       const int a = segs[q].first, b = segs[q].second, cc = segs[q + 1].second;
-      GemmDesc d{};
+      MGemm d{};
       d.M = cc - b; d.N = b - a; d.K = b - a;
-      d.a_sel = 0; d.a_off = (int64_t)a * r + b; d.lda = r;        // L[b:c, a:b]
-      d.b_sel = 1; d.b_off = (int64_t)a * r + a; d.ldb = r;        // X[a:b, a:b]
-      d.c_sel = 2; d.c_off = soff; d.ldc = cc - b;                 // scratch
+      d.a_sel = 0; d.a_off = (int64_t)a * r + b; d.lda = r;  // L[b:c, a:b]
+      d.b_sel = 0; d.b_off = (int64_t)a * r + a; d.ldb = r;  // X[a:b, a:b]
+      d.c_sel = 1; d.c_off = soff; d.ldc = cc - b;          // scratch
       d.alpha = 1.0;
+      d.tri = 1;
       add_gemm(d);
       pairs.emplace_back(a, b, cc, soff);
       soff += (int64_t)(cc - b) * (b - a);
       next.emplace_back(a, cc);
     }
     if (segs.size() % 2) next.push_back(segs.back());
-    g1.ndesc = (int)descs.size() - g1.desc_begin;
     g1.ntile = (int)tdesc.size() - g1.tile_begin;
-    FactorState::GemmLevel g2{(int)descs.size(), 0, (int)tdesc.size(), 0};
+    FactorState::InvLaunch g2{(int)tdesc.size(), 0};
     for (auto& pr : pairs) {
       const int a = std::get<0>(pr), b = std::get<1>(pr), cc = std::get<2>(pr);
-      const int64_t so = std::get<3>(pr);
-      GemmDesc d{};
+      MGemm d{};
       d.M = cc - b; d.N = b - a; d.K = cc - b;
-      d.a_sel = 1; d.a_off = (int64_t)b * r + b; d.lda = r;        // X[b:c, b:c]
-      d.b_sel = 2; d.b_off = so; d.ldb = cc - b;                   // scratch
-      d.c_sel = 1; d.c_off = (int64_t)a * r + b; d.ldc = r;        // X[b:c, a:b]
+      d.a_sel = 0; d.a_off = (int64_t)b * r + b; d.lda = r;  // X[b:c, b:c]
+      d.b_sel = 1; d.b_off = std::get<3>(pr); d.ldb = cc - b;  // scratch
+      d.c_sel = 0; d.c_off = (int64_t)a * r + b; d.ldc = r;  // X[b:c, a:b]
       d.alpha = -1.0;
+      d.tri = 2;
       add_gemm(d);
     }
-    g2.ndesc = (int)descs.size() - g2.desc_begin;
     g2.ntile = (int)tdesc.size() - g2.tile_begin;
-    fs->gemm_levels.push_back(g1);
-    fs->gemm_levels.push_back(g2);
+    fs->inv_launches.push_back(g1);
+    fs->inv_launches.push_back(g2);
     scratch_need = std::max(scratch_need, soff);
     segs.swap(next);
   }
-  fs->gemm_descs.upload(descs, st);
-  fs->gemm_tile_desc.upload(tdesc, st);
-  fs->gemm_tile_mn.upload(tmn, st);
+  fs->inv_descs.upload(descs, st);
+  fs->inv_tile_desc.upload(tdesc, st);
+  fs->inv_tile_mn.upload(tmn, st);
   const size_t rr = (size_t)r * r;
-  if (c->precision == PSG_MIXED) {
-    fs->tail_Ld32.alloc(rr); fs->tail_X32.alloc(rr); fs->tail_S32.alloc(std::max<int64_t>(1, scratch_need));
-    fs->tail_Ld64.release(); fs->tail_X64.release(); fs->tail_S64.release();
+  if (precision == PSG_MIXED) {
+    fs->tail_L32.alloc(rr); fs->tail_S32.alloc(scratch_need);
+    fs->tail_L64.release(); fs->tail_S64.release();
   } else {
-    fs->tail_Ld64.alloc(rr); fs->tail_X64.alloc(rr); fs->tail_S64.alloc(std::max<int64_t>(1, scratch_need));
-    fs->tail_Ld32.release(); fs->tail_X32.release(); fs->tail_S32.release();
+    fs->tail_L64.alloc(rr); fs->tail_S64.alloc(scratch_need);
+    fs->tail_L32.release(); fs->tail_S32.release();
   }
 }
 
+static inline int mgemm_blocks(const MGemm& g) {
+  return ((g.M + kMT - 1) / kMT) * ((g.N + kMT - 1) / kMT);
+}
+
 template <class T>
-static void tail_invert(psg_ctx* c, const T* panels, const T* dinv, T* Ld, T* X, T* Sc) {
+static void dense_factor_inverse(FactorState* fs, hipStream_t st, int r, int F, T* L, T* Sc);
+
+// The tail panels hold the tail block of D M D after every update from the
+// rest of the tree.  Gather them into L (r x r), factor it, and overwrite it
+// with X = L^-1.
+template <class T>
+static void tail_factor_dense(psg_ctx* c, const T* panels, T* L, T* Sc) {
   FactorState* fs = c->fs;
   hipStream_t st = c->stream;
-  const int r = fs->tail_r;
+  const int r = fs->tail_r, F = fs->tail_F;
   const int64_t rr = (int64_t)r * r;
-  HIP_CHECK(hipMemsetAsync(Ld, 0, rr * sizeof(T), st));
-  hipLaunchKernelGGL((k_tail_scatter<T>), dim3(r), dim3(kBlock), 0, st, r, fs->tail_F, fs->tail_s0, fs->sfirst.p,
-                     fs->rptr.p, fs->rows.p, fs->poff.p, fs->tail_chunk_of.p, panels, Ld);
+  HIP_CHECK(hipMemsetAsync(L, 0, rr * sizeof(T), st));
+  hipLaunchKernelGGL((k_tail_scatter<T>), dim3(r), dim3(kBlock), 0, st, r, F, fs->tail_s0, fs->sfirst.p,
+                     fs->rptr.p, fs->rows.p, fs->poff.p, fs->tail_chunk_of.p, panels, L);
   HIP_LAUNCH_CHECK();
-  HIP_CHECK(hipMemsetAsync(X, 0, rr * sizeof(T), st));
-  hipLaunchKernelGGL((k_tail_diag<T>), dim3(fs->tail_nchunk), dim3(kBlock), 0, st, r, fs->tail_F, fs->tail_nchunk,
-                     fs->tail_s0, fs->sfirst.p, fs->doff.p, dinv, X);
-  HIP_LAUNCH_CHECK();
-  for (const auto& g : fs->gemm_levels) {
+  dense_factor_inverse<T>(fs, st, r, F, L, Sc);
+}
+
+// Cholesky of the r x r block in L (lower part; the upper part must be zero)
+// followed by X = L^-1 over it.  A failed pivot is reported as F + column.
+template <class T>
+static void dense_factor_inverse(FactorState* fs, hipStream_t st, int r, int F, T* L, T* Sc) {
+  for (const auto& t : fs->tail_steps) {
+    if (t.kind == 0) {
+      hipLaunchKernelGGL((k_dense_potrf<T>), dim3(1), dim3(kBlock), 0, st, L, r, t.j, t.w, F, fs->fail.p);
+    } else if (t.kind == 1) {
+      const int rows = r - t.j - t.w;
+      hipLaunchKernelGGL((k_dense_trsm<T>), dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, st, L, r, t.j,
+                         t.w);
+    } else {
+      const int ntm = (t.g.M + kMT - 1) / kMT;
+      hipLaunchKernelGGL((k_mfma_gemm<T>), dim3(mgemm_blocks(t.g)), dim3(kBlock), 0, st, t.g, ntm,
+                         (const MGemm*)nullptr, (const int*)nullptr, (const int*)nullptr, L, Sc);
+    }
+    HIP_LAUNCH_CHECK();
+  }
+  for (const auto& g : fs->inv_launches) {
     if (!g.ntile) continue;
-    hipLaunchKernelGGL((k_gemm_batched<T>), dim3(g.ntile), dim3(kBlock), 0, st, fs->gemm_descs.p,
-                       fs->gemm_tile_desc.p + g.tile_begin, fs->gemm_tile_mn.p + 2 * (int64_t)g.tile_begin, Ld, X, Sc);
+    hipLaunchKernelGGL((k_mfma_gemm<T>), dim3(g.ntile), dim3(kBlock), 0, st, MGemm{}, 1, fs->inv_descs.p,
+                       fs->inv_tile_desc.p + g.tile_begin, fs->inv_tile_mn.p + 2 * (int64_t)g.tile_begin, L, Sc);
     HIP_LAUNCH_CHECK();
   }
 }
@@ -1222,21 +1411,72 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
     const int64_t w = P.sfirst[k + 1] - P.sfirst[k];
     doff[k + 1] = doff[k] + w * w;
   }
-  // unit and tile lists per level
+  const char* mode = getenv("PSG_SOLVE");
+  fs->solve_mode = (mode && mode[0] == 'l') ? 0 : 1;
+  fs->tail_r = 0;
+  fs->tail_nchunk = 0;
+  // dense tail: a postorder suffix [s0, q] of the subtree of a root q is
+  // closed under "parent of", so its columns [F, E) form the trailing
+  // diagonal block of L with nothing else below it.  The longest suffix
+  // whose block holds at least a fraction PSG_TAIL_DENSITY of nonzeros is
+  // inverted explicitly when it has PSG_TAIL_MIN columns or more.
+  if (fs->solve_mode == 1) {
+    auto knobd = [](const char* name, double dflt) {
+      const char* v = getenv(name);
+      return v ? atof(v) : dflt;
+    };
+    const int tail_min = (int)knobd("PSG_TAIL_MIN", 2048);
+    const int tail_max = (int)knobd("PSG_TAIL_MAX", 24576);
+    const double rho = knobd("PSG_TAIL_DENSITY", 0.8);
+    std::vector<int> sub(P.nsuper, 1);  // subtree sizes (postorder: children first)
+    for (int sn = 0; sn < P.nsuper; ++sn)
+      if (P.sparent[sn] >= 0) sub[P.sparent[sn]] += sub[sn];
+    int best_q = -1, best_s0 = -1, best_r = 0;
+    for (int q = 0; q < P.nsuper; ++q) {
+      if (P.sparent[q] >= 0) continue;
+      const int E = P.sfirst[q + 1];
+      double nz = 0.0;
+      for (int sn = q; sn > q - sub[q]; --sn) {
+        const double w = P.sfirst[sn + 1] - P.sfirst[sn];
+        const double nr = (double)(P.rptr[sn + 1] - P.rptr[sn]);
+        nz += w * nr - w * (w - 1) / 2;
+        const double r = E - P.sfirst[sn];
+        if (r > tail_max) break;
+        const double dens = nz / (r * (r + 1) / 2);
+        if (dens >= rho && r > best_r) { best_r = (int)r; best_q = q; best_s0 = sn; }
+        if (dens < 0.5 * rho) break;
+      }
+    }
+    fs->tail_r = 0;
+    fs->tail_nchunk = 0;
+    if (best_q >= 0 && best_r >= tail_min && best_q > best_s0) {
+      fs->tail_r = best_r;
+      fs->tail_s0 = best_s0;
+      fs->tail_nchunk = best_q - best_s0 + 1;
+      fs->tail_F = P.sfirst[best_s0];
+    }
+  }
+  auto in_tail = [&](int sn) { return fs->tail_nchunk && sn >= fs->tail_s0 && sn < fs->tail_s0 + fs->tail_nchunk; };
+  // unit and tile lists per level; the tail's supernodes are factored as
+  // one dense block after the rest of the tree
+  std::vector<int> level_nodes;
+  level_nodes.reserve(P.nsuper);
   std::vector<int> unit_s, unit_k, sunit_s, sunit_k, tile_s, tile_i, tile_j;
   std::vector<int> sunit_first(P.nsuper + 1, 0);
   fs->levels.assign(P.nlevels, {});
   fs->single_node_level.assign(P.nlevels, 0);
   for (int l = 0; l < P.nlevels; ++l) {
     LevelRange& L = fs->levels[l];
-    L.node_begin = P.level_ptr[l];
-    L.node_end = P.level_ptr[l + 1];
+    L.node_begin = (int)level_nodes.size();
+    for (int q = P.level_ptr[l]; q < P.level_ptr[l + 1]; ++q)
+      if (!in_tail(P.level_nodes[q])) level_nodes.push_back(P.level_nodes[q]);
+    L.node_end = (int)level_nodes.size();
     fs->single_node_level[l] = (L.node_end - L.node_begin) == 1;
     L.unit_begin = (int)unit_s.size();
     L.sunit_begin = (int)sunit_s.size();
     L.tile_begin = (int)tile_s.size();
     for (int q = L.node_begin; q < L.node_end; ++q) {
-      const int sn = P.level_nodes[q];
+      const int sn = level_nodes[q];
       const int w = P.sfirst[sn + 1] - P.sfirst[sn];
       const int64_t nb = (P.rptr[sn + 1] - P.rptr[sn]) - w;
       const int64_t nblk = (nb + kRB - 1) / kRB;
@@ -1284,48 +1524,6 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
       if (P.sparent[sn] >= 0) level[P.sparent[sn]] = std::max(level[P.sparent[sn]], level[sn] + 1);
     for (int sn = P.nsuper - 1; sn >= 0; --sn)
       if (P.sparent[sn] >= 0) depth[sn] = depth[P.sparent[sn]] + 1;
-    // dense tail: a postorder suffix [s0, q] of the subtree of a root q is
-    // closed under "parent of", so its columns [F, E) form the trailing
-    // diagonal block of L with nothing else below it.  The longest suffix
-    // whose block holds at least a fraction PSG_TAIL_DENSITY of nonzeros is
-    // inverted explicitly when it has PSG_TAIL_MIN columns or more.
-    {
-      auto knobd = [](const char* name, double dflt) {
-        const char* v = getenv(name);
-        return v ? atof(v) : dflt;
-      };
-      const int tail_min = (int)knobd("PSG_TAIL_MIN", 2048);
-      const int tail_max = (int)knobd("PSG_TAIL_MAX", 24576);
-      const double rho = knobd("PSG_TAIL_DENSITY", 0.8);
-      std::vector<int> sub(P.nsuper, 1);  // subtree sizes (postorder: children first)
-      for (int sn = 0; sn < P.nsuper; ++sn)
-        if (P.sparent[sn] >= 0) sub[P.sparent[sn]] += sub[sn];
-      int best_q = -1, best_s0 = -1, best_r = 0;
-      for (int q = 0; q < P.nsuper; ++q) {
-        if (P.sparent[q] >= 0) continue;
-        const int E = P.sfirst[q + 1];
-        double nz = 0.0;
-        for (int sn = q; sn > q - sub[q]; --sn) {
-          const double w = P.sfirst[sn + 1] - P.sfirst[sn];
-          const double nr = (double)(P.rptr[sn + 1] - P.rptr[sn]);
-          nz += w * nr - w * (w - 1) / 2;
-          const double r = E - P.sfirst[sn];
-          if (r > tail_max) break;
-          const double dens = nz / (r * (r + 1) / 2);
-          if (dens >= rho && r > best_r) { best_r = (int)r; best_q = q; best_s0 = sn; }
-          if (dens < 0.5 * rho) break;
-        }
-      }
-      fs->tail_r = 0;
-      fs->tail_nchunk = 0;
-      if (best_q >= 0 && best_r >= tail_min && best_q > best_s0) {
-        fs->tail_r = best_r;
-        fs->tail_s0 = best_s0;
-        fs->tail_nchunk = best_q - best_s0 + 1;
-        fs->tail_F = P.sfirst[best_s0];
-      }
-    }
-    auto in_tail = [&](int sn) { return fs->tail_nchunk && sn >= fs->tail_s0 && sn < fs->tail_s0 + fs->tail_nchunk; };
     // backward depth counts from the tail: the tail is solved before the
     // backward kernel starts, so its children begin the dependency chains
     if (fs->tail_nchunk)
@@ -1426,8 +1624,6 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
       fs->sf_part64.alloc((size_t)fs->sf_units * kW);
       fs->sf_part32.release();
     }
-    const char* mode = getenv("PSG_SOLVE");
-    fs->solve_mode = (mode && mode[0] == 'l') ? 0 : 1;
     auto knob = [](const char* name, int dflt, int lo, int hi) {
       const char* v = getenv(name);
       int x = v ? atoi(v) : dflt;
@@ -1443,7 +1639,7 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
   fs->sfirst.upload(P.sfirst, s);
   fs->rows.upload(P.rows, s);
   fs->snode_of.upload(P.snode_of, s);
-  fs->level_nodes.upload(P.level_nodes, s);
+  fs->level_nodes.upload(level_nodes, s);
   fs->rptr.upload(P.rptr, s);
   fs->poff.upload(P.poff, s);
   fs->doff.upload(doff, s);
@@ -1518,6 +1714,7 @@ static int factor_run(psg_ctx* c, const Plan& P, double eta, T* panels, T* dinv)
   for (int l = 0; l < fs->nlev; ++l) {
     const LevelRange& L = fs->levels[l];
     const int nn = L.node_end - L.node_begin;
+    if (!nn) continue;  // a level of the dense tail only
     hipLaunchKernelGGL((k_potrf<T>), dim3(nn), dim3(kBlock), 0, s, fs->level_nodes.p + L.node_begin,
                        fs->sfirst.p, fs->rptr.p, fs->poff.p, fs->doff.p, panels, dinv, fs->fail.p);
     HIP_LAUNCH_CHECK();
@@ -1540,15 +1737,15 @@ static int factor_run(psg_ctx* c, const Plan& P, double eta, T* panels, T* dinv)
       HIP_LAUNCH_CHECK();
     }
   }
+  if (fs->tail_nchunk) {
+    if constexpr (sizeof(T) == 8)
+      tail_factor_dense<double>(c, panels, fs->tail_L64.p, fs->tail_S64.p);
+    else
+      tail_factor_dense<float>(c, panels, fs->tail_L32.p, fs->tail_S32.p);
+  }
   int failcol = INT_MAX;
   HIP_CHECK(hipMemcpyAsync(&failcol, fs->fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
-  if (failcol == INT_MAX && fs->tail_nchunk && fs->solve_mode == 1) {
-    if constexpr (sizeof(T) == 8)
-      tail_invert<double>(c, panels, dinv, fs->tail_Ld64.p, fs->tail_X64.p, fs->tail_S64.p);
-    else
-      tail_invert<float>(c, panels, dinv, fs->tail_Ld32.p, fs->tail_X32.p, fs->tail_S32.p);
-  }
   return failcol == INT_MAX ? 0 : failcol + 1;
 }
 
@@ -1668,9 +1865,9 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   }
   if (fs->tail_nchunk) {
     if constexpr (sizeof(T) == 8)
-      tail_solve<double>(c, fs->tail_X64.p, vb, vy, vx);
+      tail_solve<double>(c, fs->tail_L64.p, vb, vy, vx);
     else
-      tail_solve<float>(c, fs->tail_X32.p, vb, vy, vx);
+      tail_solve<float>(c, fs->tail_L32.p, vb, vy, vx);
   }
   if (fs->sf_units) {
     hipLaunchKernelGGL((k_bwd_sf<T>), dim3(grid), dim3(kBlock), 0, s, a);
@@ -1758,6 +1955,39 @@ extern "C" int psg_solve_trace(psg_ctx* c, int64_t* out, int64_t cap, int64_t* n
     if (!c || !out || !nunits) throw Error(PSG_ERR_ARGUMENT, "psg_solve_trace: null argument");
     *nunits = factor_trace(c, reinterpret_cast<long long*>(out), cap);
     return PSG_OK;
+  });
+}
+
+extern "C" int psg_dense_cholinv(psg_ctx* c, int64_t n, double* a) {
+  return guarded([&]() -> int {
+    if (!c || !a || n <= 0 || n > 65536) throw Error(PSG_ERR_ARGUMENT, "psg_dense_cholinv: bad argument");
+    FactorState tmp;
+    const int r = (int)n;
+    tmp.tail_r = r;
+    setup_dense_schedule(&tmp, r, c->precision, c->stream);
+    tmp.fail.alloc(1);
+    const int big = INT_MAX;
+    HIP_CHECK(hipMemcpyAsync(tmp.fail.p, &big, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const size_t rr = (size_t)r * r;
+    std::vector<double> h(a, a + rr);
+    for (int j = 0; j < r; ++j)
+      for (int i = 0; i < j; ++i) h[(size_t)j * r + i] = 0.0;  // upper part must be zero
+    if (c->precision == PSG_MIXED) {
+      std::vector<float> hf(h.begin(), h.end());
+      HIP_CHECK(hipMemcpyAsync(tmp.tail_L32.p, hf.data(), rr * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      dense_factor_inverse<float>(&tmp, c->stream, r, 0, tmp.tail_L32.p, tmp.tail_S32.p);
+      HIP_CHECK(hipMemcpyAsync(hf.data(), tmp.tail_L32.p, rr * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+      for (size_t k = 0; k < rr; ++k) a[k] = hf[k];
+    } else {
+      HIP_CHECK(hipMemcpyAsync(tmp.tail_L64.p, h.data(), rr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      dense_factor_inverse<double>(&tmp, c->stream, r, 0, tmp.tail_L64.p, tmp.tail_S64.p);
+      HIP_CHECK(hipMemcpyAsync(a, tmp.tail_L64.p, rr * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    int fail = INT_MAX;
+    HIP_CHECK(hipMemcpyAsync(&fail, tmp.fail.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return fail == INT_MAX ? 0 : fail + 1;
   });
 }
 

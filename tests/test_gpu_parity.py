@@ -239,11 +239,35 @@ def test_single_launch_solve_matches_level_scheduled(gpu, monkeypatch, precision
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n", [64, 100, 300, 700, 1000, 2049])
+def test_dense_cholesky_inverse_matches_lapack(gpu, precision, n):
+    """Blocked MFMA Cholesky + recursive inversion of a dense SPD block
+    against numpy's Cholesky and triangular inverse."""
+    _say(f"dense cholinv {precision} n={n}")
+    rng = np.random.default_rng(n)
+    G = rng.standard_normal((n, n + 7))
+    S = G @ G.T / n + np.eye(n)
+    ref = la.solve_triangular(np.linalg.cholesky(S), np.eye(n), lower=True)
+    _, lp = _lp(400, 1200, 5, 3)
+    obj = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, precision=precision)
+    X = np.asfortranarray(S.copy())
+    buf = X.reshape(-1, order="F").copy()
+    assert obj._call("psg_dense_cholinv", obj._ctx, n, buf) == 0
+    X = buf.reshape(n, n, order="F")
+    err = np.abs(X - ref).max() / np.abs(ref).max()
+    print(f"dense cholinv {precision} n={n} rel err {err:.2e}", flush=True)
+    assert np.abs(np.triu(X, 1)).max() == 0.0
+    assert err <= (1e-12 if precision == "fp64" else 1e-4)
+    obj.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("m,n,k,frac,tail_min", [(1500, 4500, 5, 1.0, 64), (1500, 4500, 5, 1.0, 700),
-                                                 (3000, 9000, 6, 0.6, 128)])
+                                                 (3000, 9000, 6, 0.6, 128), (6000, 18000, 5, 1.0, 2048)])
 def test_dense_tail_inverse_solve(gpu, monkeypatch, precision, m, n, k, frac, tail_min):
-    """The explicit inverse of the dense top of the tree gives the same P^-1 r
-    as substitution (level-scheduled solves) and as LAPACK."""
+    """The dense top of the tree, factored by the blocked MFMA Cholesky and
+    inverted explicitly, gives the same P^-1 r as substitution with the
+    supernodal factor (level-scheduled solves) and as LAPACK."""
     _say(f"dense-tail solve {precision} m={m} core={frac} tail_min={tail_min}")
     _, lp = _lp(m, n, k, 71)
     A = lp.A
