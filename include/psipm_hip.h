@@ -131,6 +131,25 @@ int psg_solve_finish(psg_ctx* ctx, double* dx, double* dy);
 /* final explicit residual rhs - H dy of the last PCG run (KrylovResult.residual) */
 int psg_pcg_residual(psg_ctx* ctx, double* residual);
 
+/* ---- two right-hand sides (block PCG) ---------------------------------------
+ * No single counterpart in the reference, which solves its independent
+ * right-hand sides one after another (the start-point pair PS:4173-4188; the
+ * predictor and the centring direction of pc = 4, see ipm.py).  Arrays hold
+ * the two columns interleaved: r1 is n x 2 row-major (entry (j, q) at 2 j + q),
+ * r2, dy are m x 2, dx is n x 2.  Every read of A, of the factor's panels and
+ * of the block inverses serves both columns. */
+/* rhs_q = r2_q - A (W r1_q)  (SSP:654-655 per column) */
+int psg_solve_begin2(psg_ctx* ctx, const double* r1, const double* r2);
+/* SSP:469-581 per column with independent scalars: tol[2] per column;
+ * result[2]; alphas/betas (2 x maxit) and history (2 x (maxit + 2)), nullable */
+int psg_pcg2(psg_ctx* ctx, const double* tol, int32_t maxit, int32_t replacement, double gap_tol,
+             psg_pcg_result* result, double* alphas, double* betas, double* history);
+/* dy_q = PCG solution of column q, dx_q = W (r1_q + A^T dy_q)  (SSP:715-716) */
+int psg_solve_finish2(psg_ctx* ctx, double* dx, double* dy);
+/* H and P^-1 on two interleaved columns (m x 2) */
+int psg_apply_H2(psg_ctx* ctx, const double* v, double* out);
+int psg_apply_P2(psg_ctx* ctx, const double* r, double* z);
+
 /* ---- timing ---------------------------------------------------------------- */
 /* Device time of the hot-path pieces, measured with HIP events on the
  * library's own stream: out[0] = H v (both gathers, FP64 values),
@@ -158,6 +177,44 @@ int psg_dense_cholinv(psg_ctx* ctx, int64_t n, double* a);
  * supernodes, dense-tail order r (0: none), dense-tail supernodes, nsuper}.
  * No counterpart in the reference. */
 int psg_factor_info(psg_ctx* ctx, double* out);
+/* ---- learned column selector ---------------------------------------------
+ * No counterpart in the reference, whose core is chosen by the leverage rule
+ * alone (SAP:293-371, replaced on the host by psg_scores + the same retention
+ * procedure).  A per-column MLP 8 -> 64 -> 64 -> 1 (ReLU) scores candidate
+ * columns from their interior-point trajectory; the preconditioner object
+ * uses the scores in place of the leverage scores and falls back to the rule
+ * when they do not pay.  Parameters: float32, layout W1 (8 x 64 row-major),
+ * b1 (64), W2 (64 x 64 row-major), b2 (64), w3 (64), b3 (1); norm = feature
+ * mean[8] then 1/std[8].  Features of column j, in order: log10 theta_j at
+ * the current and the two previous psg_set_weights calls, log10 x_j,
+ * log10 z_j, log10 ||a_j||_2, log10 mu, iteration index. */
+typedef struct psg_mlp psg_mlp;
+int psg_mlp_create(psg_mlp** out);
+int psg_mlp_destroy(psg_mlp* net);
+/* number of float parameters (4801) */
+int psg_mlp_num_params(void);
+/* set parameters and normalisation; resets the Adam state */
+int psg_mlp_set_params(psg_mlp* net, const float* params, const float* norm);
+int psg_mlp_get_params(psg_mlp* net, float* params);
+/* logits of B raw feature rows (B x 8): BF16 MFMA, FP32 accumulation */
+int psg_mlp_forward(psg_mlp* net, const float* X, int64_t B, float* logits);
+/* FP32 mean weighted binary cross-entropy of the logits against labels y
+ * (pos_weight on the positive term) and its gradient (4801 floats) */
+int psg_mlp_loss_grad(psg_mlp* net, const float* X, const float* y, int64_t B, float pos_weight,
+                      float* loss, float* grad);
+/* `steps` Adam steps (FP32) on minibatches of `batch` rows drawn without
+ * replacement per epoch (permutations seeded by `seed`); losses (nullable,
+ * `steps` floats) receives the minibatch losses */
+int psg_mlp_train(psg_mlp* net, const float* X, const float* y, int64_t N, int32_t batch, int32_t steps,
+                  float lr, float pos_weight, uint64_t seed, float* losses);
+/* raw features (ncand x 8) of candidate columns: theta history from the
+ * context, x and z (n doubles each) of the current iterate, mu, iteration */
+int psg_select_features(psg_ctx* ctx, const int32_t* cand, int64_t ncand, const double* x, const double* z,
+                        double mu, double iter, float* features);
+/* the same features scored by the network (logits, ncand floats) */
+int psg_select_logits(psg_ctx* ctx, psg_mlp* net, const int32_t* cand, int64_t ncand, const double* x,
+                      const double* z, double mu, double iter, float* logits);
+
 /* device time (ms, HIP events) of the last psg_pcg and last psg_factor calls */
 int psg_last_timings(psg_ctx* ctx, double* pcg_ms, double* factor_ms);
 

@@ -21,6 +21,7 @@ PSG_MIXED = 1
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
@@ -90,6 +91,24 @@ SIGNATURES = [
     ("psg_last_timings", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("psg_factor_info", C.c_int, [C.c_void_p, _f64p]),
     ("psg_dense_cholinv", C.c_int, [C.c_void_p, C.c_int64, _f64p]),
+    ("psg_solve_begin2", C.c_int, [C.c_void_p, _f64p, _f64p]),
+    ("psg_pcg2", C.c_int, [C.c_void_p, _f64p, C.c_int32, C.c_int32, C.c_double, C.POINTER(PsgPcgResult),
+                           _opt(_f64p), _opt(_f64p), _opt(_f64p)]),
+    ("psg_solve_finish2", C.c_int, [C.c_void_p, _f64p, _f64p]),
+    ("psg_apply_H2", C.c_int, [C.c_void_p, _f64p, _f64p]),
+    ("psg_apply_P2", C.c_int, [C.c_void_p, _f64p, _f64p]),
+    ("psg_mlp_create", C.c_int, [C.POINTER(C.c_void_p)]),
+    ("psg_mlp_destroy", C.c_int, [C.c_void_p]),
+    ("psg_mlp_num_params", C.c_int, []),
+    ("psg_mlp_set_params", C.c_int, [C.c_void_p, _f32p, _f32p]),
+    ("psg_mlp_get_params", C.c_int, [C.c_void_p, _f32p]),
+    ("psg_mlp_forward", C.c_int, [C.c_void_p, _f32p, C.c_int64, _f32p]),
+    ("psg_mlp_loss_grad", C.c_int, [C.c_void_p, _f32p, _f32p, C.c_int64, C.c_float, C.POINTER(C.c_float), _f32p]),
+    ("psg_mlp_train", C.c_int, [C.c_void_p, _f32p, _f32p, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_float,
+                                C.c_uint64, _opt(_f32p)]),
+    ("psg_select_features", C.c_int, [C.c_void_p, _i32p, C.c_int64, _f64p, _f64p, C.c_double, C.c_double, _f32p]),
+    ("psg_select_logits", C.c_int, [C.c_void_p, C.c_void_p, _i32p, C.c_int64, _f64p, _f64p, C.c_double, C.c_double,
+                                    _f32p]),
 ]
 
 _lib = None

@@ -30,7 +30,9 @@ namespace psg {
 // A group of G lanes owns a segment; each group handles U segments per pass
 // and issues every independent load of a stage before consuming any, so a
 // wave keeps U dependent chains (pointer -> index/value -> gather) in flight.
-template <int G, int U, class TV>
+// With NR > 1 the vectors hold NR columns interleaved (entry (i, q) at
+// i NR + q) and every matrix entry read serves all of them.
+template <int G, int U, class TV, int NR = 1>
 __global__ void __launch_bounds__(kBlock)
 k_seg_gather(int nseg, const int* __restrict__ ptr, const int* __restrict__ idx,
              const TV* __restrict__ val, const double* __restrict__ v,
@@ -56,22 +58,36 @@ k_seg_gather(int nseg, const int* __restrict__ ptr, const int* __restrict__ idx,
       ix[u] = ok ? idx[p] : 0;
       a[u] = ok ? (double)val[p] : 0.0;
     }
-    double sum[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) sum[u] = a[u] * v[ix[u]];
+    double sum[U][NR];
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      for (int p = b[u] + lane + G; p < e[u]; p += G) sum[u] += (double)val[p] * v[idx[p]];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) sum[u][q] = a[u] * v[(int64_t)ix[u] * NR + q];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      for (int p = b[u] + lane + G; p < e[u]; p += G) {
+        const double av = (double)val[p];
+        const int64_t iv = (int64_t)idx[p] * NR;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) sum[u][q] += av * v[iv + q];
+      }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
 #pragma unroll
-      for (int o = G / 2; o > 0; o >>= 1) sum[u] += __shfl_xor(sum[u], o, 64);
+      for (int q = 0; q < NR; ++q)
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) sum[u][q] += __shfl_xor(sum[u][q], o, 64);
     }
     if (lane == 0) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int64_t seg = tile + (int64_t)u * gpb + g;
-        if (seg < nseg) out[seg] = (scale ? scale[seg] : 1.0) * (sum[u] + (add ? add[seg] : 0.0));
+        if (seg < nseg) {
+          const double sc = scale ? scale[seg] : 1.0;
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+            out[seg * NR + q] = sc * (sum[u][q] + (add ? add[seg * NR + q] : 0.0));
+        }
       }
     }
   }
@@ -82,11 +98,12 @@ enum CsrMode { kModeHv = 0, kModeResidual = 1, kModeRhs = 2 };
 // Row gather over one column block of A (CSR restricted to columns
 // [c0, c1)): the slice of t it reads stays resident in every XCD's L2.
 // Blocks after the first accumulate into `out`; the last one applies the
-// epilogue and produces the fused partial sums:
+// epilogue and produces the fused partial sums (3 per column q, at
+// part[(3 q + k) * grid + block]):
 //   kModeHv       : y = sum + delta v_i          partials (v.y, v.v, y.y)
 //   kModeResidual : y = b_i - (sum + delta v_i)  partials (y.y, |y - rref|^2)
 //   kModeRhs      : y = b_i - sum                partials (y.y)
-template <int G, int U, class TV, int MODE, bool FIRST, bool LAST>
+template <int G, int U, class TV, int MODE, bool FIRST, bool LAST, int NR = 1>
 __global__ void __launch_bounds__(kBlock)
 k_csr(int m, const int* __restrict__ ptr, const int* __restrict__ idx, const TV* __restrict__ val,
       const double* __restrict__ t, const double* __restrict__ v, double delta,
@@ -95,7 +112,9 @@ k_csr(int m, const int* __restrict__ ptr, const int* __restrict__ idx, const TV*
   const int lane = threadIdx.x % G;
   const int gpb = kBlock / G;
   const int g = threadIdx.x / G;
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[3 * NR];
+#pragma unroll
+  for (int k = 0; k < 3 * NR; ++k) acc[k] = 0.0;
   for (int64_t tile = (int64_t)blockIdx.x * U * gpb; tile < m; tile += (int64_t)gridDim.x * U * gpb) {
     int b[U], e[U];
 #pragma unroll
@@ -113,50 +132,63 @@ k_csr(int m, const int* __restrict__ ptr, const int* __restrict__ idx, const TV*
       ix[u] = ok ? idx[p] : 0;
       a[u] = ok ? (double)val[p] : 0.0;
     }
-    double sum[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) sum[u] = a[u] * t[ix[u]];
+    double sum[U][NR];
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      for (int p = b[u] + lane + G; p < e[u]; p += G) sum[u] += (double)val[p] * t[idx[p]];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) sum[u][q] = a[u] * t[(int64_t)ix[u] * NR + q];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      for (int p = b[u] + lane + G; p < e[u]; p += G) {
+        const double av = (double)val[p];
+        const int64_t it = (int64_t)idx[p] * NR;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) sum[u][q] += av * t[it + q];
+      }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
 #pragma unroll
-      for (int o = G / 2; o > 0; o >>= 1) sum[u] += __shfl_xor(sum[u], o, 64);
+      for (int q = 0; q < NR; ++q)
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) sum[u][q] += __shfl_xor(sum[u][q], o, 64);
     }
     if (lane == 0) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int64_t row = tile + (int64_t)u * gpb + g;
         if (row >= m) continue;
-        double sm = sum[u];
-        if (!FIRST) sm += out[row];
-        if (!LAST) {
-          out[row] = sm;
-        } else if (MODE == kModeHv) {
-          const double vi = v[row];
-          const double y = sm + delta * vi;
-          out[row] = y;
-          acc[0] += vi * y;
-          acc[1] += vi * vi;
-          acc[2] += y * y;
-        } else if (MODE == kModeResidual) {
-          const double y = bvec[row] - (sm + delta * v[row]);
-          out[row] = y;
-          acc[0] += y * y;
-          if (rref) {
-            const double d = y - rref[row];
-            acc[1] += d * d;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          const int64_t o = row * NR + q;
+          double sm = sum[u][q];
+          if (!FIRST) sm += out[o];
+          if (!LAST) {
+            out[o] = sm;
+          } else if (MODE == kModeHv) {
+            const double vi = v[o];
+            const double y = sm + delta * vi;
+            out[o] = y;
+            acc[3 * q] += vi * y;
+            acc[3 * q + 1] += vi * vi;
+            acc[3 * q + 2] += y * y;
+          } else if (MODE == kModeResidual) {
+            const double y = bvec[o] - (sm + delta * v[o]);
+            out[o] = y;
+            acc[3 * q] += y * y;
+            if (rref) {
+              const double d = y - rref[o];
+              acc[3 * q + 1] += d * d;
+            }
+          } else {
+            const double y = bvec[o] - sm;
+            out[o] = y;
+            acc[3 * q] += y * y;
           }
-        } else {
-          const double y = bvec[row] - sm;
-          out[row] = y;
-          acc[0] += y * y;
         }
       }
     }
   }
-  if (LAST && part) write_partials<3>(acc, part);
+  if (LAST && part) write_partials<3 * NR>(acc, part);
 }
 
 // one block: reduce NP per-block partials to NP scalars (fixed order)
@@ -335,13 +367,13 @@ static int pick_group(double avg) {
   return g;
 }
 
-template <class TV>
+template <class TV, int NR = 1>
 static void launch_gather(int G, int nseg, const int* ptr, const int* idx, const TV* val,
                           const double* v, const double* scale, const double* add, double* out,
                           hipStream_t s) {
   const int grid = grid_for((int64_t)nseg * G, kBlock * kUnroll, 65535);
   switch (G) {
-#define CASE(g) case g: hipLaunchKernelGGL((k_seg_gather<g, kUnroll, TV>), dim3(grid), dim3(kBlock), 0, s, nseg, ptr, idx, val, v, scale, add, out); break;
+#define CASE(g) case g: hipLaunchKernelGGL((k_seg_gather<g, kUnroll, TV, NR>), dim3(grid), dim3(kBlock), 0, s, nseg, ptr, idx, val, v, scale, add, out); break;
     CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
 #undef CASE
     default: throw Error(PSG_ERR_INTERNAL, "bad group size");
@@ -349,13 +381,13 @@ static void launch_gather(int G, int nseg, const int* ptr, const int* idx, const
   HIP_LAUNCH_CHECK();
 }
 
-template <class TV, int MODE, bool FIRST, bool LAST>
+template <class TV, int MODE, bool FIRST, bool LAST, int NR = 1>
 static int launch_csr_block(int G, int m, const int* ptr, const int* idx, const TV* val, const double* t,
                             const double* v, double delta, const double* b, const double* rref, double* out,
                             double* part, hipStream_t s) {
   const int grid = grid_for((int64_t)m * G, kBlock * kUnroll, kSpmvBlocks);
   switch (G) {
-#define CASE(g) case g: hipLaunchKernelGGL((k_csr<g, kUnroll, TV, MODE, FIRST, LAST>), dim3(grid), dim3(kBlock), 0, s, m, ptr, idx, val, t, v, delta, b, rref, out, part); break;
+#define CASE(g) case g: hipLaunchKernelGGL((k_csr<g, kUnroll, TV, MODE, FIRST, LAST, NR>), dim3(grid), dim3(kBlock), 0, s, m, ptr, idx, val, t, v, delta, b, rref, out, part); break;
     CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
 #undef CASE
     default: throw Error(PSG_ERR_INTERNAL, "bad group size");
@@ -366,7 +398,7 @@ static int launch_csr_block(int G, int m, const int* ptr, const int* idx, const 
 
 // y = A t with the MODE epilogue, over the column blocks of the split CSR.
 // Returns the partials grid of the last launch.
-template <class TV, int MODE>
+template <class TV, int MODE, int NR = 1>
 static int launch_csr(const psg_ctx* c, const TV* val, const double* t, const double* v, double delta,
                       const double* b, const double* rref, double* out, double* part, hipStream_t s) {
   const int S = c->nsplit;
@@ -376,13 +408,13 @@ static int launch_csr(const psg_ctx* c, const TV* val, const double* t, const do
     const int* ptr = c->rptr.p + (size_t)k * (m + 1);
     const bool first = k == 0, last = k == S - 1;
     if (first && last)
-      grid = launch_csr_block<TV, MODE, true, true>(c->g_csr, m, ptr, c->rcol.p, val, t, v, delta, b, rref, out, part, s);
+      grid = launch_csr_block<TV, MODE, true, true, NR>(c->g_csr, m, ptr, c->rcol.p, val, t, v, delta, b, rref, out, part, s);
     else if (first)
-      grid = launch_csr_block<TV, MODE, true, false>(c->g_csr, m, ptr, c->rcol.p, val, t, v, delta, b, rref, out, part, s);
+      grid = launch_csr_block<TV, MODE, true, false, NR>(c->g_csr, m, ptr, c->rcol.p, val, t, v, delta, b, rref, out, part, s);
     else if (last)
-      grid = launch_csr_block<TV, MODE, false, true>(c->g_csr, m, ptr, c->rcol.p, val, t, v, delta, b, rref, out, part, s);
+      grid = launch_csr_block<TV, MODE, false, true, NR>(c->g_csr, m, ptr, c->rcol.p, val, t, v, delta, b, rref, out, part, s);
     else
-      grid = launch_csr_block<TV, MODE, false, false>(c->g_csr, m, ptr, c->rcol.p, val, t, v, delta, b, rref, out, part, s);
+      grid = launch_csr_block<TV, MODE, false, false, NR>(c->g_csr, m, ptr, c->rcol.p, val, t, v, delta, b, rref, out, part, s);
   }
   return grid;
 }
@@ -566,6 +598,7 @@ int psg_destroy(psg_ctx* c) {
   if (!c) return PSG_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   factor_destroy(c);
+  delete c->b2;
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -577,6 +610,15 @@ int psg_destroy(psg_ctx* c) {
 int psg_set_weights(psg_ctx* c, const double* theta, double delta) {
   return guarded([&]() -> int {
     if (!c || !theta) throw Error(PSG_ERR_ARGUMENT, "psg_set_weights: null argument");
+    // keep the two previous weight vectors (buffer rotation, no copies)
+    if (c->nhist >= 1) {
+      std::swap(c->theta_h2.p, c->theta_h1.p);
+      std::swap(c->theta_h2.n, c->theta_h1.n);
+      std::swap(c->theta_h1.p, c->theta.p);
+      std::swap(c->theta_h1.n, c->theta.n);
+      c->theta.grow(c->n);
+    }
+    c->nhist = std::min(3, c->nhist + 1);
     c->theta.upload(theta, c->n, c->stream);
     hipLaunchKernelGGL(k_recip, dim3(grid_for(c->n, kBlock, 4096)), dim3(kBlock), 0, c->stream, c->n,
                        c->theta.p, c->w.p);
@@ -847,6 +889,450 @@ int psg_last_timings(psg_ctx* c, double* pcg_ms, double* factor_ms) {
     if (!c) throw Error(PSG_ERR_ARGUMENT, "psg_last_timings: null context");
     if (pcg_ms) *pcg_ms = c->last_pcg_ms;
     if (factor_ms) *factor_ms = c->last_factor_ms;
+    return PSG_OK;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Two right-hand sides solved together (block PCG with independent
+// recurrences): both columns share every read of A, of the factor's panels
+// and of the block inverses.  Each column follows SSP `_pcg` exactly -- own
+// alpha/beta guards, explicit residual checks, replacement, info codes --
+// and is frozen once it stops while the other one runs on.
+// ---------------------------------------------------------------------------
+namespace psg {
+
+enum DirMode { kDirNormal = 0, kDirReset = 1, kDirFrozen = 2 };
+
+// partials of a_q . b_q for both columns (part[q * grid + block])
+__global__ void __launch_bounds__(kBlock)
+k_dot2(int m, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ part) {
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    acc[0] += a[2 * i] * b[2 * i];
+    acc[1] += a[2 * i + 1] * b[2 * i + 1];
+  }
+  write_partials<2>(acc, part);
+}
+
+// out_{i,q} = w_i r1_{i,q}
+__global__ void k_mul2(int n, const double* __restrict__ w, const double* __restrict__ r1, double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    out[2 * i] = w[i] * r1[2 * i];
+    out[2 * i + 1] = w[i] * r1[2 * i + 1];
+  }
+}
+
+// dst column q <- src column q
+__global__ void k_copy_col2(int m, int q, const double* __restrict__ src, double* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    dst[2 * i + q] = src[2 * i + q];
+}
+
+// Per column: alpha = rz / (p.q) with the guards of SSP:490-501, then
+// x += alpha p, r -= alpha q and partials of r.r.  hv = (p.q, p.p, q.q)
+// of column 0 then column 1; scal holds a kScalarHeader block per column,
+// then grid partials per column.
+__global__ void __launch_bounds__(kBlock)
+k_pcg_step2(int m, const double* __restrict__ hv, const double* __restrict__ rz0, const double* __restrict__ rz1,
+            const double* __restrict__ p, const double* __restrict__ q, double* __restrict__ x,
+            double* __restrict__ r, double* __restrict__ scal, double* __restrict__ al0, double* __restrict__ al1) {
+  double alpha[2] = {0.0, 0.0};
+  bool live[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    double* h = scal + c * kScalarHeader;
+    live[c] = h[kStop] == 0.0;
+    if (!live[c]) continue;
+    const double curvature = hv[3 * c];
+    const double floor_ = std::numeric_limits<double>::min() * fmax(1.0, sqrt(hv[3 * c + 1]) * sqrt(hv[3 * c + 2]));
+    const double a = (c == 0 ? *rz0 : *rz1) / curvature;
+    int fail = 0;
+    if (!isfinite(curvature) || curvature <= floor_) fail = -2;
+    else if (!isfinite(a) || a <= 0.0) fail = -3;
+    if (fail) {
+      live[c] = false;
+      // other blocks read h[kStop] only at entry: raise it after the barrier
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        h[kInfo] = fail;
+        h[kCurv] = curvature;
+      }
+    }
+    alpha[c] = a;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      double* h = scal + c * kScalarHeader;
+      if (h[kStop] == 0.0 && !live[c]) h[kStop] = 1.0;
+    }
+  }
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (!live[c]) continue;
+      const int64_t o = 2 * i + c;
+      x[o] += alpha[c] * p[o];
+      const double ri = r[o] - alpha[c] * q[o];
+      r[o] = ri;
+      acc[c] += ri * ri;
+    }
+  }
+  block_sum<2>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) scal[2 * kScalarHeader + c * gridDim.x + blockIdx.x] = acc[c];
+    if (blockIdx.x == 0) {
+      if (live[0]) { scal[kAlpha] = alpha[0]; scal[kCurv] = hv[0]; *al0 = alpha[0]; }
+      if (live[1]) { scal[kScalarHeader + kAlpha] = alpha[1]; scal[kScalarHeader + kCurv] = hv[3]; *al1 = alpha[1]; }
+    }
+  }
+}
+
+// Per column, by mode: normal -> rz_new = r.z, guards of SSP:545-553,
+// beta = rz_new / rz, p = z + beta p; reset (after a residual replacement)
+// -> rz_new with the guard of SSP:528-531 (info -5), p = z; frozen -> none.
+__global__ void __launch_bounds__(kBlock)
+k_pcg_direction2(int m, const double* __restrict__ partC, int nbC, const int* __restrict__ mode,
+                 const double* __restrict__ rz0, const double* __restrict__ rz1, double* __restrict__ rzn0,
+                 double* __restrict__ rzn1, const double* __restrict__ z, double* __restrict__ p,
+                 double* __restrict__ scal, double* __restrict__ be0, double* __restrict__ be1) {
+  double tot[2];
+  reduce_partials<2>(partC, nbC, tot);
+  double beta[2] = {0.0, 0.0};
+  int md[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    double* h = scal + c * kScalarHeader;
+    md[c] = (h[kStop] != 0.0) ? kDirFrozen : mode[c];
+    if (md[c] == kDirFrozen) continue;
+    const double rz_new = tot[c];
+    int fail = 0;
+    if (md[c] == kDirReset) {
+      if (!isfinite(rz_new) || rz_new <= 0.0) fail = -5;
+    } else {
+      beta[c] = rz_new / (c == 0 ? *rz0 : *rz1);
+      if (!isfinite(rz_new) || rz_new <= 0.0) fail = -6;
+      else if (!isfinite(beta[c]) || beta[c] < 0.0) fail = -7;
+    }
+    if (fail) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        h[kInfo] = fail;
+        h[kStop] = 1.0;
+      }
+      md[c] = kDirFrozen;
+      continue;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (c == 0) *rzn0 = rz_new; else *rzn1 = rz_new;
+      if (md[c] == kDirNormal) {
+        const int nb = (int)h[kNBeta];
+        (c == 0 ? be0 : be1)[nb] = beta[c];
+        h[kNBeta] = nb + 1;
+        h[kBeta] = beta[c];
+      }
+    }
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (md[c] == kDirFrozen) continue;
+      const int64_t o = 2 * i + c;
+      p[o] = (md[c] == kDirReset) ? z[o] : z[o] + beta[c] * p[o];
+    }
+  }
+}
+
+static Block2& block2(psg_ctx* c) {
+  if (!c->b2) {
+    auto* b = new Block2;
+    try {
+      for (auto* v : {&b->rhs, &b->x, &b->r, &b->z, &b->p, &b->q, &b->rt}) v->alloc(2 * (size_t)c->m);
+      for (auto* v : {&b->tn, &b->r1n, &b->dxn}) v->alloc(2 * (size_t)c->n);
+      b->partA.alloc(6 * kSpmvBlocks);
+      b->partD.alloc(6 * kSpmvBlocks);
+      b->partC.alloc(2 * kRedBlocks);
+      b->hvdots.alloc(8);
+      b->scal.alloc(2 * kScalarHeader + 2 * kRedBlocks);
+      b->mode.alloc(2);
+      HIP_CHECK(hipHostMalloc((void**)&b->hscal, (2 * kScalarHeader + 2 * kRedBlocks) * sizeof(double)));
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    c->b2 = b;
+  }
+  return *c->b2;
+}
+
+// H applied to both columns of v; partials (p.q, p.p, q.q) per column
+// finalized into dots[0..5] when dots is set
+static int device_Hv2(psg_ctx* c, Block2& b, const double* v, double* out, double* part, bool lowp, double* dots) {
+  int nb;
+  if (lowp && c->precision == PSG_MIXED) {
+    launch_gather<float, 2>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval32.p, v, c->w.p, nullptr, b.tn.p, c->stream);
+    nb = launch_csr<float, kModeHv, 2>(c, c->rval32.p, b.tn.p, v, c->delta, nullptr, nullptr, out, part, c->stream);
+  } else {
+    launch_gather<double, 2>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval.p, v, c->w.p, nullptr, b.tn.p, c->stream);
+    nb = launch_csr<double, kModeHv, 2>(c, c->rval.p, b.tn.p, v, c->delta, nullptr, nullptr, out, part, c->stream);
+  }
+  if (dots) {
+    hipLaunchKernelGGL(k_finalize<6>, dim3(1), dim3(kBlock), 0, c->stream, part, nb, dots);
+    HIP_LAUNCH_CHECK();
+  }
+  return nb;
+}
+
+// out = b - H v for both columns (FP64); partials (|y|^2, |y - rref|^2, -) per column
+static int device_residual2(psg_ctx* c, Block2& bb, const double* b, const double* v, const double* rref,
+                            double* out, double* part) {
+  launch_gather<double, 2>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval.p, v, c->w.p, nullptr, bb.tn.p, c->stream);
+  return launch_csr<double, kModeResidual, 2>(c, c->rval.p, bb.tn.p, v, c->delta, b, rref, out, part, c->stream);
+}
+
+static int device_dot2(psg_ctx* c, const double* a, const double* b, double* part) {
+  const int grid = grid_for(c->m, kBlock, kRedBlocks);
+  hipLaunchKernelGGL(k_dot2, dim3(grid), dim3(kBlock), 0, c->stream, c->m, a, b, part);
+  HIP_LAUNCH_CHECK();
+  return grid;
+}
+
+}  // namespace psg
+
+extern "C" {
+
+int psg_solve_begin2(psg_ctx* c, const double* r1, const double* r2) {
+  return guarded([&]() -> int {
+    if (!c || !r1 || !r2) throw Error(PSG_ERR_ARGUMENT, "psg_solve_begin2: null argument");
+    need_weights(c);
+    Block2& b = block2(c);
+    hipStream_t s = c->stream;
+    HIP_CHECK(hipMemcpyAsync(b.r1n.p, r1, 2 * (size_t)c->n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(b.rt.p, r2, 2 * (size_t)c->m * sizeof(double), hipMemcpyHostToDevice, s));
+    // u = W r1 ; rhs = r2 - A u  per column (SSP:654-655)
+    hipLaunchKernelGGL(k_mul2, dim3(grid_for(c->n, kBlock, 4096)), dim3(kBlock), 0, s, c->n, c->w.p, b.r1n.p, b.tn.p);
+    HIP_LAUNCH_CHECK();
+    launch_csr<double, kModeRhs, 2>(c, c->rval.p, b.tn.p, nullptr, 0.0, b.rt.p, nullptr, b.rhs.p, nullptr, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    b.have_rhs = true;
+    b.have_solution = false;
+    return PSG_OK;
+  });
+}
+
+int psg_pcg2(psg_ctx* c, const double* tol, int32_t maxit, int32_t replacement, double gap_tol,
+             psg_pcg_result* res, double* alphas_out, double* betas_out, double* history_out) {
+  return guarded([&]() -> int {
+    if (!c || !res || !tol || maxit < 1) throw Error(PSG_ERR_ARGUMENT, "psg_pcg2: invalid arguments");
+    Block2* bp = c->b2;
+    if (!bp || !bp->have_rhs) throw Error(PSG_ERR_STATE, "psg_pcg2: call psg_solve_begin2 first");
+    if (!factor_ready(c)) throw Error(PSG_ERR_STATE, "psg_pcg2: no factor");
+    Block2& b = *bp;
+    const int m = c->m;
+    hipStream_t s = c->stream;
+    const int cap = maxit + 2;
+    b.rzs.grow(2 * (size_t)cap);
+    b.alphas.grow(2 * (size_t)cap);
+    b.betas.grow(2 * (size_t)cap);
+    double* rzs[2] = {b.rzs.p, b.rzs.p + cap};
+    double* als[2] = {b.alphas.p, b.alphas.p + cap};
+    double* bes[2] = {b.betas.p, b.betas.p + cap};
+    HIP_CHECK(hipEventRecord(c->ev0, s));
+    HIP_CHECK(hipMemsetAsync(b.scal.p, 0, 2 * kScalarHeader * sizeof(double), s));
+    double tmp[6];
+    int nb = device_dot2(c, b.rhs.p, b.rhs.p, b.partD.p);
+    read_partials(c, b.partD.p, nb, 2, tmp);
+    double scale[2], tolq[2] = {tol[0], tol[1]};
+    for (int q = 0; q < 2; ++q) scale[q] = std::max(1.0, std::sqrt(tmp[q]));
+    // x0 = P^-1 rhs ; r = rhs - H x0 ; z = P^-1 r ; rz = r.z ; p = z
+    factor_apply2(c, b.rhs.p, b.x.p);
+    nb = device_residual2(c, b, b.rhs.p, b.x.p, nullptr, b.r.p, b.partD.p);
+    read_partials(c, b.partD.p, nb, 6, tmp);
+    std::vector<double> hist[2];
+    for (int q = 0; q < 2; ++q) hist[q].push_back(std::sqrt(tmp[3 * q]) / scale[q]);
+    factor_apply2(c, b.r.p, b.z.p);
+    nb = device_dot2(c, b.r.p, b.z.p, b.partD.p);
+    read_partials(c, b.partD.p, nb, 2, tmp);
+    int info[2], it[2] = {0, 0}, rz_slot[2] = {0, 0}, alpha_start[2] = {0, 0}, beta_start[2] = {0, 0};
+    int replacements[2] = {0, 0}, checks[2] = {0, 0};
+    bool done[2] = {false, false};
+    for (int q = 0; q < 2; ++q) {
+      const double rz = tmp[q];
+      HIP_CHECK(hipMemcpyAsync(rzs[q], &rz, sizeof(double), hipMemcpyHostToDevice, s));
+      info[q] = maxit;
+      if (hist[q].back() <= tolq[q]) info[q] = 0;
+      else if (!std::isfinite(rz) || rz <= 0.0) info[q] = -1;
+      done[q] = info[q] <= 0;
+    }
+    HIP_CHECK(hipMemcpyAsync(b.p.p, b.z.p, 2 * (size_t)m * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const int grid_m = grid_for(m, kBlock, kRedBlocks);
+    auto freeze = [&](int q) {
+      const double one = 1.0;
+      HIP_CHECK(hipMemcpyAsync(b.scal.p + q * kScalarHeader + kStop, &one, sizeof(double), hipMemcpyHostToDevice, s));
+      done[q] = true;
+    };
+    for (int q = 0; q < 2; ++q)
+      if (done[q]) freeze(q);
+    while (!(done[0] && done[1])) {
+      device_Hv2(c, b, b.p.p, b.q.p, b.partA.p, true, b.hvdots.p);
+      hipLaunchKernelGGL(k_pcg_step2, dim3(grid_m), dim3(kBlock), 0, s, m, b.hvdots.p, rzs[0] + rz_slot[0],
+                         rzs[1] + rz_slot[1], b.p.p, b.q.p, b.x.p, b.r.p, b.scal.p, als[0] + it[0], als[1] + it[1]);
+      HIP_LAUNCH_CHECK();
+      HIP_CHECK(hipMemcpyAsync(b.hscal, b.scal.p, (2 * kScalarHeader + 2 * grid_m) * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      bool need_check[2] = {false, false};
+      for (int q = 0; q < 2; ++q) {
+        if (done[q]) continue;
+        const double* h = b.hscal + q * kScalarHeader;
+        if (h[kStop] != 0.0) {
+          info[q] = (int)h[kInfo];
+          done[q] = true;
+          continue;
+        }
+        ++it[q];
+        const double rr = host_sum(b.hscal + 2 * kScalarHeader + q * grid_m, grid_m);
+        const double rec = std::sqrt(rr) / scale[q];
+        const bool replace = replacement > 0 && it[q] % replacement == 0;
+        need_check[q] = replace || rec <= tolq[q];
+      }
+      int mode[2] = {kDirNormal, kDirNormal};
+      if (need_check[0] || need_check[1]) {
+        nb = device_residual2(c, b, b.rhs.p, b.x.p, b.r.p, b.rt.p, b.partD.p);
+        read_partials(c, b.partD.p, nb, 6, tmp);
+        for (int q = 0; q < 2; ++q) {
+          if (!need_check[q]) continue;
+          ++checks[q];
+          const double tn = std::sqrt(tmp[3 * q]);
+          const double rel = tn / scale[q];
+          hist[q].push_back(rel);
+          if (!std::isfinite(rel)) {
+            info[q] = -4;
+            freeze(q);
+            continue;
+          }
+          if (rel <= tolq[q]) {
+            hipLaunchKernelGGL(k_copy_col2, dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, q, b.rt.p, b.r.p);
+            HIP_LAUNCH_CHECK();
+            info[q] = 0;
+            freeze(q);
+            continue;
+          }
+          const double gap = std::sqrt(tmp[3 * q + 1]);
+          if (gap > gap_tol * std::max(tn, tolq[q] * scale[q])) {
+            hipLaunchKernelGGL(k_copy_col2, dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, q, b.rt.p, b.r.p);
+            HIP_LAUNCH_CHECK();
+            ++replacements[q];
+            mode[q] = kDirReset;
+            alpha_start[q] = it[q];
+            beta_start[q] = (int)b.hscal[q * kScalarHeader + kNBeta];
+          }
+        }
+      }
+      if (done[0] && done[1]) break;
+      for (int q = 0; q < 2; ++q) {
+        if (!done[q] && it[q] >= maxit) {
+          freeze(q);  // iteration limit: info stays maxit (> 0)
+        }
+      }
+      if (done[0] && done[1]) break;
+      HIP_CHECK(hipMemcpyAsync(b.mode.p, mode, 2 * sizeof(int), hipMemcpyHostToDevice, s));
+      factor_apply2(c, b.r.p, b.z.p);
+      const int nbC = device_dot2(c, b.r.p, b.z.p, b.partC.p);
+      hipLaunchKernelGGL(k_pcg_direction2, dim3(grid_m), dim3(kBlock), 0, s, m, b.partC.p, nbC, b.mode.p,
+                         rzs[0] + rz_slot[0], rzs[1] + rz_slot[1], rzs[0] + it[0], rzs[1] + it[1], b.z.p, b.p.p,
+                         b.scal.p, bes[0], bes[1]);
+      HIP_LAUNCH_CHECK();
+      for (int q = 0; q < 2; ++q)
+        if (!done[q]) rz_slot[q] = it[q];
+    }
+    // pick up direction-kernel failures raised after the last synchronisation
+    HIP_CHECK(hipMemcpyAsync(b.hscal, b.scal.p, 2 * kScalarHeader * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    int nbeta_total[2];
+    for (int q = 0; q < 2; ++q) {
+      const double* h = b.hscal + q * kScalarHeader;
+      if (info[q] > 0 && h[kStop] != 0.0 && h[kInfo] != 0.0) info[q] = (int)h[kInfo];
+      nbeta_total[q] = (int)h[kNBeta];
+    }
+    // final explicit residuals (SSP:558-561), kept in rt
+    nb = device_residual2(c, b, b.rhs.p, b.x.p, nullptr, b.rt.p, b.partD.p);
+    read_partials(c, b.partD.p, nb, 6, tmp);
+    HIP_CHECK(hipEventRecord(c->ev1, s));
+    for (int q = 0; q < 2; ++q) {
+      const double relative = std::sqrt(tmp[3 * q]) / scale[q];
+      if (info[q] > 0 && relative <= tolq[q]) info[q] = 0;
+      const int na = std::max(0, it[q] - alpha_start[q]);
+      const int nbt = std::max(0, nbeta_total[q] - beta_start[q]);
+      if (alphas_out && na)
+        HIP_CHECK(hipMemcpyAsync(alphas_out + (size_t)q * maxit, als[q] + alpha_start[q], na * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+      if (betas_out && nbt)
+        HIP_CHECK(hipMemcpyAsync(betas_out + (size_t)q * maxit, bes[q] + beta_start[q], nbt * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+      res[q].info = info[q];
+      res[q].iterations = it[q];
+      res[q].relative_residual = relative;
+      res[q].replacements = replacements[q];
+      res[q].residual_checks = checks[q];
+      res[q].n_alpha = na;
+      res[q].n_beta = nbt;
+      res[q].n_history = (int)std::min<size_t>(hist[q].size(), (size_t)cap);
+      res[q].stagnated = 0;
+      if (history_out) std::copy(hist[q].begin(), hist[q].begin() + res[q].n_history, history_out + (size_t)q * cap);
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_pcg_ms = ms;
+    if (factor_solve_error(c)) throw Error(PSG_ERR_INTERNAL, "triangular solve dependency wait timed out");
+    b.have_solution = true;
+    return PSG_OK;
+  });
+}
+
+int psg_solve_finish2(psg_ctx* c, double* dx, double* dy) {
+  return guarded([&]() -> int {
+    if (!c || !dx || !dy) throw Error(PSG_ERR_ARGUMENT, "psg_solve_finish2: null argument");
+    if (!c->b2 || !c->b2->have_solution) throw Error(PSG_ERR_STATE, "psg_solve_finish2: no PCG solution");
+    Block2& b = *c->b2;
+    // dx = W (r1 + A^T dy) per column (SSP:716)
+    launch_gather<double, 2>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval.p, b.x.p, c->w.p, b.r1n.p, b.dxn.p,
+                             c->stream);
+    HIP_CHECK(hipMemcpyAsync(dx, b.dxn.p, 2 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(dy, b.x.p, 2 * (size_t)c->m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PSG_OK;
+  });
+}
+
+int psg_apply_H2(psg_ctx* c, const double* v, double* out) {
+  return guarded([&]() -> int {
+    if (!c || !v || !out) throw Error(PSG_ERR_ARGUMENT, "psg_apply_H2: null argument");
+    need_weights(c);
+    Block2& b = block2(c);
+    HIP_CHECK(hipMemcpyAsync(b.z.p, v, 2 * (size_t)c->m * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    device_Hv2(c, b, b.z.p, b.q.p, b.partA.p, false, nullptr);
+    HIP_CHECK(hipMemcpyAsync(out, b.q.p, 2 * (size_t)c->m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PSG_OK;
+  });
+}
+
+int psg_apply_P2(psg_ctx* c, const double* r, double* z) {
+  return guarded([&]() -> int {
+    if (!c || !r || !z) throw Error(PSG_ERR_ARGUMENT, "psg_apply_P2: null argument");
+    if (!factor_ready(c)) throw Error(PSG_ERR_STATE, "psg_apply_P2: no factor");
+    Block2& b = block2(c);
+    HIP_CHECK(hipMemcpyAsync(b.z.p, r, 2 * (size_t)c->m * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    factor_apply2(c, b.z.p, b.q.p);
+    HIP_CHECK(hipMemcpyAsync(z, b.q.p, 2 * (size_t)c->m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (factor_solve_error(c)) throw Error(PSG_ERR_INTERNAL, "triangular solve dependency wait timed out");
     return PSG_OK;
   });
 }

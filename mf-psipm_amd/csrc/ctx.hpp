@@ -18,6 +18,21 @@ enum ScalarSlot {
   kScalarHeader = 16,  // partials of the last axpy follow the header
 };
 
+// State of the two-right-hand-side PCG (psg_pcg2): vectors hold the two
+// columns interleaved (entry (i, q) at 2 i + q); per-column scalar blocks
+// of kScalarHeader doubles precede the per-block partials in `scal`.
+struct Block2 {
+  DevBuf<double> rhs, x, r, z, p, q, rt;  // 2 m
+  DevBuf<double> tn, r1n, dxn;            // 2 n
+  DevBuf<double> partA, partC, partD, hvdots, scal, rzs, alphas, betas;
+  DevBuf<int> mode;                       // per-column direction mode
+  double* hscal = nullptr;                // pinned mirror of scal
+  bool have_rhs = false, have_solution = false;
+  ~Block2() {
+    if (hscal) (void)hipHostFree(hscal);
+  }
+};
+
 }  // namespace psg
 
 struct psg_ctx {
@@ -34,6 +49,10 @@ struct psg_ctx {
   int nsplit = 1;             // column blocks of the CSR copy (rptr holds nsplit x (m+1))
   // Newton matrix
   psg::DevBuf<double> theta, w;
+  // weights of the two previous psg_set_weights calls (learned selector
+  // features); nhist counts the calls, capped at 3
+  psg::DevBuf<double> theta_h1, theta_h2;
+  int nhist = 0;
   double delta = 0.0;
   bool have_weights = false;
   // n-vectors
@@ -44,6 +63,9 @@ struct psg_ctx {
   psg::DevBuf<uint8_t> mask;
   psg::DevBuf<int> cand;
   psg::DevBuf<double> cand_out;
+  // learned selector inputs/outputs
+  psg::DevBuf<double> sel_x, sel_z;
+  psg::DevBuf<float> sel_feat, sel_out;
   // reductions and PCG scalars
   psg::DevBuf<double> partA, partC, partD;  // per-block partials
   psg::DevBuf<double> hvdots;               // p.q, p.p, q.q of the last H p
@@ -54,6 +76,7 @@ struct psg_ctx {
   double last_pcg_ms = 0.0, last_factor_ms = 0.0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   psg::FactorState* fs = nullptr;
+  psg::Block2* b2 = nullptr;
 };
 
 namespace psg {
@@ -63,6 +86,8 @@ int factor_numeric(psg_ctx* ctx, const Plan& plan, double eta, double* nnz_facto
 bool factor_ready(const psg_ctx* ctx);
 // z = D P^T L^-T L^-1 P D r on device vectors (m), on ctx->stream
 void factor_apply(psg_ctx* ctx, const double* r, double* z);
+// the same for two right-hand sides, interleaved (entry (i, q) at 2 i + q)
+void factor_apply2(psg_ctx* ctx, const double* r, double* z);
 // nonzero if a dependency wait of the last single-launch solve timed out
 int factor_solve_error(psg_ctx* ctx);
 // context.hip helpers used by factor.hip

@@ -23,6 +23,7 @@ constexpr int kBlock = 256;       // threads per workgroup (4 wavefronts of 64)
 constexpr int kWave = 64;
 constexpr int kRedBlocks = 512;   // grid of reduction-producing vector kernels
 constexpr int kSpmvBlocks = 2048; // grid cap of the row-gather kernels (partials finalized separately)
+constexpr int kMaxRhs = 2;        // right-hand sides solved together (block PCG)
 
 // RAII device buffer
 template <class T>

@@ -387,21 +387,26 @@ k_update(const int* __restrict__ tile_s, const int* __restrict__ tile_i, const i
 // ---------------------------------------------------------------------------
 // triangular solves
 // ---------------------------------------------------------------------------
-template <class T>
+// Solve vectors hold NR right-hand sides interleaved: entry (k, q) at k NR + q.
+template <class T, int NR = 1>
 __global__ void k_perm_in(int m, const int* __restrict__ perm, const double* __restrict__ dsc,
                           const double* __restrict__ r, T* __restrict__ b) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x) {
     const int i = perm[k];
-    b[k] = (T)(dsc[i] * r[i]);
+    const double d = dsc[i];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) b[k * NR + q] = (T)(d * r[(int64_t)i * NR + q]);
   }
 }
 
-template <class T>
+template <class T, int NR = 1>
 __global__ void k_perm_out(int m, const int* __restrict__ perm, const double* __restrict__ dsc,
                            const T* __restrict__ x, double* __restrict__ z) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x) {
     const int i = perm[k];
-    z[i] = dsc[i] * (double)x[k];
+    const double d = dsc[i];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) z[(int64_t)i * NR + q] = d * (double)x[k * NR + q];
   }
 }
 
@@ -608,11 +613,11 @@ __device__ inline T row_dot(const T* __restrict__ P, int64_t nr, const T* ys, in
   return (acc0 + acc1) + (acc2 + acc3);
 }
 
-template <class T>
+template <class T, int NR = 1>
 __global__ void __launch_bounds__(kBlock) k_fwd_sf(SolveArgs<T> a) {
   __shared__ T sD[kW][kW + 1];
-  __shared__ T bs[kW], ys[kW];
-  __shared__ T ypart[kBlock / kWave][kW];
+  __shared__ T bs[NR][kW], ys[NR][kW];
+  __shared__ T ypart[NR][kBlock / kWave][kW];
   __shared__ int tgt[kBlock];
   __shared__ int sh_u;
   const int tid = threadIdx.x;
@@ -653,36 +658,47 @@ __global__ void __launch_bounds__(kBlock) k_fwd_sf(SolveArgs<T> a) {
       if (a.trace) a.trace[6 * (int64_t)u + 1] = wall_clock64();
     }
     __syncthreads();
-    if (tid < kW) bs[tid] = (tid < w) ? load_agent(a.b + f + tid) : T(0);
+    if (tid < kW) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) bs[q][tid] = (tid < w) ? load_agent(a.b + (int64_t)(f + tid) * NR + q) : T(0);
+    }
     __syncthreads();
     // ys = L_ss^-1 b_s: wave wv sums columns [16 wv, 16 wv + 16) for row `lane`
-    {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
       T acc = T(0);
 #pragma unroll
       for (int k = 0; k < kW / 4; ++k) {
         const int c = wv * (kW / 4) + k;
-        acc += sD[lane][c] * bs[c];
+        acc += sD[lane][c] * bs[q][c];
       }
-      ypart[wv][lane] = acc;
+      ypart[q][wv][lane] = acc;
     }
     __syncthreads();
     if (tid < kW) {
-      const T y = (tid < w) ? (ypart[0][tid] + ypart[1][tid]) + (ypart[2][tid] + ypart[3][tid]) : T(0);
-      ys[tid] = y;
-      if (first && tid < w) a.y[f + tid] = y;  // read by the backward launch
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        const T y = (tid < w) ? (ypart[q][0][tid] + ypart[q][1][tid]) + (ypart[q][2][tid] + ypart[q][3][tid]) : T(0);
+        ys[q][tid] = y;
+        if (first && tid < w) a.y[(int64_t)(f + tid) * NR + q] = y;  // read by the backward launch
+      }
     }
     __syncthreads();
     if (tid < nrow) {
-      T acc0 = T(0), acc1 = T(0), acc2 = T(0), acc3 = T(0);
+      // the panel row is read once for all NR right-hand sides
 #pragma unroll
-      for (int c = 0; c < kW; c += 4) {
-        acc0 += v[c] * ys[c];
-        acc1 += v[c + 1] * ys[c + 1];
-        acc2 += v[c + 2] * ys[c + 2];
-        acc3 += v[c + 3] * ys[c + 3];
+      for (int q = 0; q < NR; ++q) {
+        T acc0 = T(0), acc1 = T(0), acc2 = T(0), acc3 = T(0);
+#pragma unroll
+        for (int c = 0; c < kW; c += 4) {
+          acc0 += v[c] * ys[q][c];
+          acc1 += v[c + 1] * ys[q][c + 1];
+          acc2 += v[c + 2] * ys[q][c + 2];
+          acc3 += v[c + 3] * ys[q][c + 3];
+        }
+        __hip_atomic_fetch_add(a.b + (int64_t)row * NR + q, -((acc0 + acc1) + (acc2 + acc3)), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
       }
-      __hip_atomic_fetch_add(a.b + row, -((acc0 + acc1) + (acc2 + acc3)), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
     }
     tgt[tid] = t;
     __builtin_amdgcn_s_waitcnt(0);  // this wave's atomics performed before the barrier
@@ -729,12 +745,12 @@ __device__ inline T transpose_reduce16(T (&acc)[16], int lane) {
   return acc[0];
 }
 
-template <class T>
+template <class T, int NR = 1>
 __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
   __shared__ T sD[kW][kW + 1];
-  __shared__ T xr[kBlock];
-  __shared__ T cs[kW];
-  __shared__ T part_sh[kBlock / kWave][kWave];
+  __shared__ T xr[NR][kBlock];
+  __shared__ T cs[NR][kW];
+  __shared__ T part_sh[NR][kBlock / kWave][kWave];
   __shared__ int tg[kBlock];
   __shared__ int sh_u, sh_last;
   const int tid = threadIdx.x;
@@ -786,25 +802,31 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
     if (tid < nrow && (tid == 0 || tg[tid - 1] != tg[tid])) wait_at_least(&a.xdone[tg[tid]], 1, a.err, a.sleep_max);
     __syncthreads();
     if (a.trace && tid == 0) a.trace[6 * (int64_t)u + 4] = wall_clock64();
-    xr[tid] = (tid < nrow) ? load_agent(a.x + row) : T(0);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) xr[q][tid] = (tid < nrow) ? load_agent(a.x + (int64_t)row * NR + q) : T(0);
     __syncthreads();
-    // partial[c] = sum over the wave's rows of L[r][c] x[row]
-    {
+    // partial[c] = sum over the wave's rows of L[r][c] x[row]; the panel
+    // block in registers serves every right-hand side
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
       T acc[16];
 #pragma unroll
       for (int j = 0; j < 16; ++j) acc[j] = T(0);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const T xk = xr[i0 + rq + 16 * k];
+        const T xk = xr[q][i0 + rq + 16 * k];
 #pragma unroll
         for (int j = 0; j < 16; ++j) acc[j] += v[k][j] * xk;
       }
-      part_sh[wv][cg + 4 * rq] = transpose_reduce16(acc, lane);
+      part_sh[q][wv][cg + 4 * rq] = transpose_reduce16(acc, lane);
     }
     __syncthreads();
     if (tid < w && nrow > 0) {
-      const T sum = (part_sh[0][tid] + part_sh[1][tid]) + (part_sh[2][tid] + part_sh[3][tid]);
-      __hip_atomic_fetch_add(a.acc + f + tid, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        const T sum = (part_sh[q][0][tid] + part_sh[q][1][tid]) + (part_sh[q][2][tid] + part_sh[q][3][tid]);
+        __hip_atomic_fetch_add(a.acc + (int64_t)(f + tid) * NR + q, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -817,13 +839,18 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
     if (sh_last) {
       stage_inverse(sD, a.dinv + a.doff[s], w);
       if (tid < w) {
-        cs[tid] = a.y[f + tid] - load_agent(a.acc + f + tid);
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+          cs[q][tid] = a.y[(int64_t)(f + tid) * NR + q] - load_agent(a.acc + (int64_t)(f + tid) * NR + q);
       }
       __syncthreads();
       if (tid < w) {
-        T acc = T(0);
-        for (int j = tid; j < w; ++j) acc += sD[j][tid] * cs[j];
-        store_agent(a.x + f + tid, acc);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          T acc = T(0);
+          for (int j = tid; j < w; ++j) acc += sD[j][tid] * cs[q][j];
+          store_agent(a.x + (int64_t)(f + tid) * NR + q, acc);
+        }
       }
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
@@ -842,7 +869,7 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
 // wide bottom levels (thousands of small supernodes) run at full occupancy
 // instead of one workgroup per unit.
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, int NR = 1>
 __global__ void __launch_bounds__(kBlock)
 k_fwd_low(int count, const int* __restrict__ nodes, const int* __restrict__ sfirst, const int64_t* __restrict__ rptr,
           const int* __restrict__ rows, const int64_t* __restrict__ poff, const int64_t* __restrict__ doff,
@@ -856,36 +883,54 @@ k_fwd_low(int count, const int* __restrict__ nodes, const int* __restrict__ sfir
   const int64_t nr = rptr[s + 1] - rptr[s];
   const T* D = dinv + doff[s];
   // y_s = L_ss^-1 b_s, lane j owns y_j
-  const T bj = (lane < w) ? b[f + lane] : T(0);
-  T yj = T(0);
-  for (int k = 0; k < w; ++k) {
-    const T bk = __shfl(bj, k);
-    if (lane < w) yj += D[(int64_t)k * w + lane] * bk;
+  T bj[NR], yj[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    bj[q] = (lane < w) ? b[(int64_t)(f + lane) * NR + q] : T(0);
+    yj[q] = T(0);
   }
-  if (lane < w) y[f + lane] = yj;
+  for (int k = 0; k < w; ++k) {
+    const T dk = (lane < w) ? D[(int64_t)k * w + lane] : T(0);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) yj[q] += dk * __shfl(bj[q], k);
+  }
+  if (lane < w) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) y[(int64_t)(f + lane) * NR + q] = yj[q];
+  }
   // b[rows below] -= L_below y_s, lane = row
   const T* P = panels + poff[s];
   const int* R = rows + rptr[s];
   for (int64_t i0 = w; i0 < nr; i0 += kWave) {
     const int64_t i = i0 + lane;
-    T acc0 = T(0), acc1 = T(0);
+    T acc0[NR], acc1[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc0[q] = acc1[q] = T(0);
     int c = 0;
     for (; c + 1 < w; c += 2) {
-      const T y0 = __shfl(yj, c), y1 = __shfl(yj, c + 1);
-      if (i < nr) {
-        acc0 += P[(int64_t)c * nr + i] * y0;
-        acc1 += P[(int64_t)(c + 1) * nr + i] * y1;
+      const T p0 = (i < nr) ? P[(int64_t)c * nr + i] : T(0);
+      const T p1 = (i < nr) ? P[(int64_t)(c + 1) * nr + i] : T(0);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        acc0[q] += p0 * __shfl(yj[q], c);
+        acc1[q] += p1 * __shfl(yj[q], c + 1);
       }
     }
     if (c < w) {
-      const T y0 = __shfl(yj, c);
-      if (i < nr) acc0 += P[(int64_t)c * nr + i] * y0;
+      const T p0 = (i < nr) ? P[(int64_t)c * nr + i] : T(0);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) acc0[q] += p0 * __shfl(yj[q], c);
     }
-    if (i < nr) __hip_atomic_fetch_add(b + R[i], -(acc0 + acc1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (i < nr) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q)
+        __hip_atomic_fetch_add(b + (int64_t)R[i] * NR + q, -(acc0[q] + acc1[q]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
-template <class T>
+template <class T, int NR = 1>
 __global__ void __launch_bounds__(kBlock)
 k_bwd_low(int count, const int* __restrict__ nodes, const int* __restrict__ sfirst, const int64_t* __restrict__ rptr,
           const int* __restrict__ rows, const int64_t* __restrict__ poff, const int64_t* __restrict__ doff,
@@ -902,33 +947,52 @@ k_bwd_low(int count, const int* __restrict__ nodes, const int* __restrict__ sfir
   // acc_c = sum_{below rows} L[i][c] x[row_i]; lane (rq, cg) takes rows
   // rq + 16k of each 64-row chunk and columns cg + 4j
   const int rq = lane & 15, cg = lane >> 4;
-  T acc[16];
+  T acc[NR][16];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) acc[j] = T(0);
+  for (int q = 0; q < NR; ++q)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[q][j] = T(0);
   for (int64_t i0 = w; i0 < nr; i0 += kWave) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int64_t i = i0 + rq + 16 * k;
       if (i < nr) {
-        const T xk = x[R[i]];
+        T xk[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) xk[q] = x[(int64_t)R[i] * NR + q];
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
           const int c = cg + 4 * j;
-          if (c < w) acc[j] += P[(int64_t)c * nr + i] * xk;
+          if (c < w) {
+            const T pv = P[(int64_t)c * nr + i];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc[q][j] += pv * xk[q];
+          }
         }
       }
     }
   }
-  const T red = transpose_reduce16(acc, lane);  // column cg + 4 rq
   const int c_own = cg + 4 * rq;
-  const T t = (c_own < w) ? y[f + c_own] - red : T(0);
-  // x_s = L_ss^-T t: lane j sums D(c, j) t_c over c >= j
-  T xj = T(0);
-  for (int c = 0; c < w; ++c) {
-    const T tc = __shfl(t, (c >> 2) + 16 * (c & 3));
-    if (lane < w && c >= lane) xj += dinv[doff[s] + (int64_t)lane * w + c] * tc;
+  T t[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const T red = transpose_reduce16(acc[q], lane);  // column cg + 4 rq
+    t[q] = (c_own < w) ? y[(int64_t)(f + c_own) * NR + q] - red : T(0);
   }
-  if (lane < w) x[f + lane] = xj;
+  // x_s = L_ss^-T t: lane j sums D(c, j) t_c over c >= j
+  T xj[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) xj[q] = T(0);
+  for (int c = 0; c < w; ++c) {
+    const int src = (c >> 2) + 16 * (c & 3);
+    const T dv = (lane < w && c >= lane) ? dinv[doff[s] + (int64_t)lane * w + c] : T(0);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) xj[q] += dv * __shfl(t[q], src);
+  }
+  if (lane < w) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) x[(int64_t)(f + lane) * NR + q] = xj[q];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1144,44 +1208,64 @@ k_dense_trsm(T* __restrict__ L, int r, int j, int w) {
 
 // y_T += X b_T over tiles (256 rows x 512 columns, lower part only)
 constexpr int kTR = 256, kTC = 512;
-template <class T>
+template <class T, int NR = 1>
 __global__ void __launch_bounds__(kBlock)
 k_tail_lower(int r, const int* __restrict__ tiles, const T* __restrict__ X, const T* __restrict__ b,
              T* __restrict__ y) {
-  __shared__ T bs[kTC];
+  __shared__ T bs[NR][kTC];
   const int I = tiles[2 * blockIdx.x], J = tiles[2 * blockIdx.x + 1];
   const int j0 = J * kTC;
-  for (int q = threadIdx.x; q < kTC; q += blockDim.x) bs[q] = (j0 + q < r) ? b[j0 + q] : T(0);
+  for (int q = threadIdx.x; q < kTC; q += blockDim.x) {
+#pragma unroll
+    for (int e = 0; e < NR; ++e) bs[e][q] = (j0 + q < r) ? b[(int64_t)(j0 + q) * NR + e] : T(0);
+  }
   __syncthreads();
   const int i = I * kTR + threadIdx.x;
   if (i >= r) return;
   const int jend = min(min(j0 + kTC, r), i + 1);
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[NR][4];
+#pragma unroll
+  for (int e = 0; e < NR; ++e) acc[e][0] = acc[e][1] = acc[e][2] = acc[e][3] = 0.0;
   int j = j0;
   for (; j + 3 < jend; j += 4) {
     const T x0 = X[(int64_t)j * r + i], x1 = X[(int64_t)(j + 1) * r + i];
     const T x2 = X[(int64_t)(j + 2) * r + i], x3 = X[(int64_t)(j + 3) * r + i];
-    acc[0] += (double)x0 * bs[j - j0];
-    acc[1] += (double)x1 * bs[j + 1 - j0];
-    acc[2] += (double)x2 * bs[j + 2 - j0];
-    acc[3] += (double)x3 * bs[j + 3 - j0];
+#pragma unroll
+    for (int e = 0; e < NR; ++e) {
+      acc[e][0] += (double)x0 * bs[e][j - j0];
+      acc[e][1] += (double)x1 * bs[e][j + 1 - j0];
+      acc[e][2] += (double)x2 * bs[e][j + 2 - j0];
+      acc[e][3] += (double)x3 * bs[e][j + 3 - j0];
+    }
   }
-  for (; j < jend; ++j) acc[0] += (double)X[(int64_t)j * r + i] * bs[j - j0];
-  const double sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-  if (jend > j0) __hip_atomic_fetch_add(y + i, (T)sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (; j < jend; ++j) {
+    const T x0 = X[(int64_t)j * r + i];
+#pragma unroll
+    for (int e = 0; e < NR; ++e) acc[e][0] += (double)x0 * bs[e][j - j0];
+  }
+  if (jend > j0) {
+#pragma unroll
+    for (int e = 0; e < NR; ++e) {
+      const double sum = (acc[e][0] + acc[e][1]) + (acc[e][2] + acc[e][3]);
+      __hip_atomic_fetch_add(y + (int64_t)i * NR + e, (T)sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
 
 // x_T += X' y_T over tiles (64 columns x 1024 rows): a wave owns a column,
 // its lanes stream the column's contiguous rows
 constexpr int kUC = 64, kUR = 1024;
-template <class T>
+template <class T, int NR = 1>
 __global__ void __launch_bounds__(kBlock)
 k_tail_upper(int r, const int* __restrict__ tiles, const T* __restrict__ X, const T* __restrict__ y,
              T* __restrict__ x) {
-  __shared__ T ys[kUR];
+  __shared__ T ys[NR][kUR];
   const int J = tiles[2 * blockIdx.x], I = tiles[2 * blockIdx.x + 1];
   const int i0 = I * kUR;
-  for (int q = threadIdx.x; q < kUR; q += blockDim.x) ys[q] = (i0 + q < r) ? y[i0 + q] : T(0);
+  for (int q = threadIdx.x; q < kUR; q += blockDim.x) {
+#pragma unroll
+    for (int e = 0; e < NR; ++e) ys[e][q] = (i0 + q < r) ? y[(int64_t)(i0 + q) * NR + e] : T(0);
+  }
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   for (int cc = wv; cc < kUC; cc += kBlock / kWave) {
@@ -1189,11 +1273,20 @@ k_tail_upper(int r, const int* __restrict__ tiles, const T* __restrict__ X, cons
     if (j >= r) break;
     const int ibeg = max(i0, j), iend = min(i0 + kUR, r);
     if (ibeg >= iend) continue;
-    double acc = 0.0;
+    double acc[NR];
+#pragma unroll
+    for (int e = 0; e < NR; ++e) acc[e] = 0.0;
     const T* col = X + (int64_t)j * r;
-    for (int i = ibeg + lane; i < iend; i += kWave) acc += (double)col[i] * ys[i - i0];
-    acc = wave_sum(acc);
-    if (lane == 0) __hip_atomic_fetch_add(x + j, (T)acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = ibeg + lane; i < iend; i += kWave) {
+      const double xv = (double)col[i];
+#pragma unroll
+      for (int e = 0; e < NR; ++e) acc[e] += xv * ys[e][i - i0];
+    }
+#pragma unroll
+    for (int e = 0; e < NR; ++e) {
+      const double sm = wave_sum(acc[e]);
+      if (lane == 0) __hip_atomic_fetch_add(x + (int64_t)j * NR + e, (T)sm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
@@ -1378,18 +1471,18 @@ static void dense_factor_inverse(FactorState* fs, hipStream_t st, int r, int F, 
   }
 }
 
-template <class T>
+template <class T, int NR = 1>
 static void tail_solve(psg_ctx* c, const T* X, T* vb, T* vy, T* vx) {
   FactorState* fs = c->fs;
   hipStream_t st = c->stream;
   const int r = fs->tail_r, F = fs->tail_F;
-  HIP_CHECK(hipMemsetAsync(vy + F, 0, (size_t)r * sizeof(T), st));
-  hipLaunchKernelGGL((k_tail_lower<T>), dim3(fs->tail_nlower), dim3(kBlock), 0, st, r, fs->tail_lower_tiles.p, X,
-                     vb + F, vy + F);
+  HIP_CHECK(hipMemsetAsync(vy + (size_t)F * NR, 0, (size_t)r * NR * sizeof(T), st));
+  hipLaunchKernelGGL((k_tail_lower<T, NR>), dim3(fs->tail_nlower), dim3(kBlock), 0, st, r, fs->tail_lower_tiles.p, X,
+                     vb + (size_t)F * NR, vy + (size_t)F * NR);
   HIP_LAUNCH_CHECK();
-  HIP_CHECK(hipMemsetAsync(vx + F, 0, (size_t)r * sizeof(T), st));
-  hipLaunchKernelGGL((k_tail_upper<T>), dim3(fs->tail_nupper), dim3(kBlock), 0, st, r, fs->tail_upper_tiles.p, X,
-                     vy + F, vx + F);
+  HIP_CHECK(hipMemsetAsync(vx + (size_t)F * NR, 0, (size_t)r * NR * sizeof(T), st));
+  hipLaunchKernelGGL((k_tail_upper<T, NR>), dim3(fs->tail_nupper), dim3(kBlock), 0, st, r, fs->tail_upper_tiles.p, X,
+                     vy + (size_t)F * NR, vx + (size_t)F * NR);
   HIP_LAUNCH_CHECK();
   // tail supernodes count as solved for the backward dependency waits
   hipLaunchKernelGGL(k_fill_int, dim3(grid_for(fs->tail_nchunk, kBlock, 1024)), dim3(kBlock), 0, st,
@@ -1665,16 +1758,16 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
   if (c->precision == PSG_MIXED) {
     fs->panels32.alloc(std::max<int64_t>(1, fs->npanel));
     fs->dinv32.alloc(std::max<int64_t>(1, fs->ndinv));
-    fs->vb32.alloc(2 * (size_t)P.m);
-    for (auto* b : {&fs->vy32, &fs->vx32}) b->alloc(P.m);
+    fs->vb32.alloc(2 * (size_t)P.m * kMaxRhs);
+    for (auto* b : {&fs->vy32, &fs->vx32}) b->alloc((size_t)P.m * kMaxRhs);
     fs->part32.alloc(npart);
     fs->panels64.release();
     fs->dinv64.release();
   } else {
     fs->panels64.alloc(std::max<int64_t>(1, fs->npanel));
     fs->dinv64.alloc(std::max<int64_t>(1, fs->ndinv));
-    fs->vb64.alloc(2 * (size_t)P.m);
-    for (auto* b : {&fs->vy64, &fs->vx64}) b->alloc(P.m);
+    fs->vb64.alloc(2 * (size_t)P.m * kMaxRhs);
+    for (auto* b : {&fs->vy64, &fs->vx64}) b->alloc((size_t)P.m * kMaxRhs);
     fs->part64.alloc(npart);
     fs->panels32.release();
     fs->dinv32.release();
@@ -1789,20 +1882,23 @@ static void apply_run(psg_ctx* c, const double* r, double* z, const T* panels, c
 
 
 // isolated 1x1 supernodes: x_k = dinv_k^2 b_k (forward and backward at once)
-template <class T>
+template <class T, int NR = 1>
 __global__ void k_iso_solve(int n, const int* __restrict__ col, const int* __restrict__ doff,
                             const T* __restrict__ dinv, const T* __restrict__ b, T* __restrict__ y,
                             T* __restrict__ x) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int k = col[i];
     const T d = dinv[doff[i]];
-    const T yk = d * b[k];
-    y[k] = yk;
-    x[k] = d * yk;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const T yk = d * b[(int64_t)k * NR + q];
+      y[(int64_t)k * NR + q] = yk;
+      x[(int64_t)k * NR + q] = d * yk;
+    }
   }
 }
 
-template <class T>
+template <class T, int NR = 1>
 static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels, const T* dinv, T* vb, T* vy,
                          T* vx, T* part) {
   FactorState* fs = c->fs;
@@ -1810,7 +1906,7 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   const int m = c->m;
   const int ns = fs->nsuper;
   HIP_CHECK(hipMemsetAsync(fs->flags.p, 0, fs->flags.n * sizeof(int), s));
-  hipLaunchKernelGGL((k_perm_in<T>), dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, fs->perm.p, c->dsc.p,
+  hipLaunchKernelGGL((k_perm_in<T, NR>), dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, fs->perm.p, c->dsc.p,
                      r, vb);
   HIP_LAUNCH_CHECK();
   SolveArgs<T> a;
@@ -1839,14 +1935,14 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   a.xdone = fs->flags.p + 3 * ns;
   a.ticket = fs->flags.p + 4 * ns;
   a.err = fs->flags.p + 4 * ns + 4;
-  a.acc = vb + m;  // second half of the solve buffer, zeroed below
+  a.acc = vb + (size_t)m * NR;  // second half of the solve buffer, zeroed below
   a.sleep_max = fs->sf_sleep;
   a.batch = fs->sf_batch;
   a.trace = fs->trace.p;
-  HIP_CHECK(hipMemsetAsync(a.acc, 0, (size_t)m * sizeof(T), s));
+  HIP_CHECK(hipMemsetAsync(a.acc, 0, (size_t)m * NR * sizeof(T), s));
   const int grid = std::max(1, std::min(fs->sf_units, fs->sf_grid));
   if (fs->n_iso) {
-    hipLaunchKernelGGL((k_iso_solve<T>), dim3(grid_for(fs->n_iso, kBlock, 4096)), dim3(kBlock), 0, s, fs->n_iso,
+    hipLaunchKernelGGL((k_iso_solve<T, NR>), dim3(grid_for(fs->n_iso, kBlock, 4096)), dim3(kBlock), 0, s, fs->n_iso,
                        fs->iso_col.p, fs->iso_doff.p, dinv, vb, vy, vx);
     HIP_LAUNCH_CHECK();
   }
@@ -1854,33 +1950,33 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   const int wpb = kBlock / kWave;
   for (int L = 0; L < nlow; ++L) {
     const int cnt_l = fs->low_ptr[L + 1] - fs->low_ptr[L];
-    hipLaunchKernelGGL((k_fwd_low<T>), dim3((cnt_l + wpb - 1) / wpb), dim3(kBlock), 0, s, cnt_l,
+    hipLaunchKernelGGL((k_fwd_low<T, NR>), dim3((cnt_l + wpb - 1) / wpb), dim3(kBlock), 0, s, cnt_l,
                        fs->low_nodes.p + fs->low_ptr[L], fs->sfirst.p, fs->rptr.p, fs->rows.p, fs->poff.p,
                        fs->doff.p, panels, dinv, vb, vy);
     HIP_LAUNCH_CHECK();
   }
   if (fs->sf_units) {
-    hipLaunchKernelGGL((k_fwd_sf<T>), dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_fwd_sf<T, NR>), dim3(grid), dim3(kBlock), 0, s, a);
     HIP_LAUNCH_CHECK();
   }
   if (fs->tail_nchunk) {
     if constexpr (sizeof(T) == 8)
-      tail_solve<double>(c, fs->tail_L64.p, vb, vy, vx);
+      tail_solve<double, NR>(c, fs->tail_L64.p, vb, vy, vx);
     else
-      tail_solve<float>(c, fs->tail_L32.p, vb, vy, vx);
+      tail_solve<float, NR>(c, fs->tail_L32.p, vb, vy, vx);
   }
   if (fs->sf_units) {
-    hipLaunchKernelGGL((k_bwd_sf<T>), dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_bwd_sf<T, NR>), dim3(grid), dim3(kBlock), 0, s, a);
     HIP_LAUNCH_CHECK();
   }
   for (int L = nlow - 1; L >= 0; --L) {
     const int cnt_l = fs->low_ptr[L + 1] - fs->low_ptr[L];
-    hipLaunchKernelGGL((k_bwd_low<T>), dim3((cnt_l + wpb - 1) / wpb), dim3(kBlock), 0, s, cnt_l,
+    hipLaunchKernelGGL((k_bwd_low<T, NR>), dim3((cnt_l + wpb - 1) / wpb), dim3(kBlock), 0, s, cnt_l,
                        fs->low_nodes.p + fs->low_ptr[L], fs->sfirst.p, fs->rptr.p, fs->rows.p, fs->poff.p,
                        fs->doff.p, panels, dinv, vy, vx);
     HIP_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL((k_perm_out<T>), dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, fs->perm.p,
+  hipLaunchKernelGGL((k_perm_out<T, NR>), dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, fs->perm.p,
                      c->dsc.p, vx, z);
   HIP_LAUNCH_CHECK();
 }
@@ -1944,6 +2040,17 @@ void factor_apply(psg_ctx* c, const double* r, double* z) {
     apply_run<float>(c, r, z, fs->panels32.p, fs->dinv32.p, fs->vb32.p, fs->vy32.p, fs->vx32.p, fs->part32.p);
   else
     apply_run<double>(c, r, z, fs->panels64.p, fs->dinv64.p, fs->vb64.p, fs->vy64.p, fs->vx64.p, fs->part64.p);
+}
+
+// Two right-hand sides at once (vectors interleaved, entry (i, q) at 2 i + q):
+// every panel entry and block inverse is read once for both.
+void factor_apply2(psg_ctx* c, const double* r, double* z) {
+  FactorState* fs = c->fs;
+  if (fs->solve_mode != 1) throw Error(PSG_ERR_STATE, "two-column solves need the single-launch solve layout");
+  if (c->precision == PSG_MIXED)
+    apply_run_sf<float, 2>(c, r, z, fs->panels32.p, fs->dinv32.p, fs->vb32.p, fs->vy32.p, fs->vx32.p, fs->sf_part32.p);
+  else
+    apply_run_sf<double, 2>(c, r, z, fs->panels64.p, fs->dinv64.p, fs->vb64.p, fs->vy64.p, fs->vx64.p, fs->sf_part64.p);
 }
 
 }  // namespace psg

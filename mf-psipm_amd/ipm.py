@@ -169,6 +169,33 @@ def prox_eval(
         dw[ibox] = (mw[ibox] + w[ibox] * dx[ibox]) / s[ibox]
         return dx, dy, dz, dw
 
+    def newton_pair(first, second):
+        """Two Newton directions of the same matrix (pc = 4): one block solve
+        when the KKT object offers ``solve_pair``, two solves otherwise."""
+        r1s = []
+        for rd, _, mz, mw in (first, second):
+            r1 = rd.copy()
+            r1[ipos] += mz[ipos] / x[ipos]
+            r1[ibox] -= mw[ibox] / s[ibox]
+            r1s.append(r1)
+        try:
+            if hasattr(kkt, "solve_pair"):
+                sols = kkt.solve_pair(r1s[0], first[1], r1s[1], second[1])
+            else:
+                sols = (kkt.solve(r1s[0], first[1]), kkt.solve(r1s[1], second[1]))
+        except (KKTSolveError, np.linalg.LinAlgError, FloatingPointError):
+            return None
+        out = []
+        for (dx, dy), (_, _, mz, mw) in zip(sols, (first, second)):
+            if not (np.isfinite(dx).all() and np.isfinite(dy).all()):
+                return None
+            dz = np.zeros(n)
+            dw = np.zeros(n)
+            dz[ipos] = (mz[ipos] - z[ipos] * dx[ipos]) / x[ipos]
+            dw[ibox] = (mw[ibox] + w[ibox] * dx[ibox]) / s[ibox]
+            out.append((dx, dy, dz, dw))
+        return out
+
     def primal_step(d):
         return tau * min(step_to_boundary(x, d, ipos), step_to_boundary(s, -d, ibox))
 
@@ -212,6 +239,26 @@ def prox_eval(
                 rmw[ibox] = sigma * mu - s[ibox] * w[ibox]
                 direction = newton(res_d, res_p, rmz, rmw)
                 sigma_used = sigma
+            elif pc == 4:
+                # predictor and pure-centring direction from one block solve;
+                # Mehrotra's sigma = (mu_aff / mu)^3 combines them (the
+                # second-order term of the corrector is dropped)
+                rmz[ipos] = -x[ipos] * z[ipos]
+                rmw[ibox] = -s[ibox] * w[ibox]
+                cz, cw = np.zeros(n), np.zeros(n)
+                cz[ipos] = mu
+                cw[ibox] = mu
+                pair = newton_pair((res_d, res_p, rmz, rmw), (zero_d, zero_p, cz, cw))
+                if pair is not None:
+                    aff, cen = pair
+                    ax = primal_step(aff[0])
+                    az = dual_step(aff[2], aff[3])
+                    val = (x[ipos] + ax * aff[0][ipos]) @ (z[ipos] + az * aff[2][ipos])
+                    val += (s[ibox] - ax * aff[0][ibox]) @ (w[ibox] + az * aff[3][ibox])
+                    mu_aff = val / npos_box if npos_box else 0.0
+                    sigma = min(1.0, max(0.0, (mu_aff / mu) ** 3)) if mu > 0 else 0.0
+                    direction = tuple(a + sigma * cc for a, cc in zip(aff, cen))
+                    sigma_used = sigma
             else:
                 if pc == 2:
                     rmz[ipos] = -x[ipos] * z[ipos]
@@ -381,8 +428,13 @@ def starting_point(A, At, b, c, u, lp: StandardLP, ipos, ibox, solver,
     else:
         try:
             solver.factorize(np.ones(n), 1.0)
-            x = At @ solver.solve(np.zeros(n), b)[1]
-            y = solver.solve(np.zeros(n), A @ c)[1]
+            if hasattr(solver, "solve_pair"):
+                # the two start systems are independent: one block solve
+                (_, dyx), (_, dyy) = solver.solve_pair(np.zeros(n), b, np.zeros(n), A @ c)
+                x, y = At @ dyx, dyy
+            else:
+                x = At @ solver.solve(np.zeros(n), b)[1]
+                y = solver.solve(np.zeros(n), A @ c)[1]
         except Exception:
             x, y = np.zeros(n), np.zeros(m)
     if relaxed and saved is not None:

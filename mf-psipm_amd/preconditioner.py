@@ -115,6 +115,7 @@ class HipSpectralStableColumnPreconditioner:
         minres_max_dimension=200_000, fail_tol=1e-6, low_rank_max_columns=8,
         low_rank_min_nnz=1024, low_rank_clique_ratio=2.0, ordering=1, max_width=64,
         dense_factor=10.0, max_factor_entries: Optional[float] = None, error_class=KKTSolveError,
+        selector=None, selector_patience=2, recorder=None,
         **_ignored,
     ):
         if core_factor <= 0.0:
@@ -236,6 +237,20 @@ class HipSpectralStableColumnPreconditioner:
         self.residual_replacements = self.residual_checks = self.stagnation_events = 0
         self.core_swaps = self.minres_fallbacks = self.minres_iterations = 0
         self.krylov_breakdowns = self.krylov_iteration_limits = 0
+        # learned column selection (no counterpart in the reference): the
+        # network's scores replace the leverage scores at core refreshes until
+        # `selector_patience` learned refreshes in a row are followed by a
+        # poor solve; then the rule takes over (fallback)
+        self.selector = selector
+        self.selector_patience = max(1, int(selector_patience))
+        self.recorder = recorder
+        self._learned_active = selector is not None
+        self._learned_pending = False
+        self._learned_strikes = 0
+        self._obs = None
+        self._iteration = 0
+        self.learned_refreshes = self.learned_fallbacks = 0
+        self.pair_solves = self.pair_fallbacks = 0
         # device-time accounting (HIP events)
         self.device_pcg_ms = 0.0
         self.device_factor_ms = 0.0
@@ -288,7 +303,12 @@ class HipSpectralStableColumnPreconditioner:
         self._rhs_index = 0
 
     def observe(self, x, z, w, s, mu):
-        """mu-based forcing capped by the EW estimate (SSP:252-258)."""
+        """mu-based forcing capped by the EW estimate (SSP:252-258).
+
+        The iterate is also kept (by reference) for the learned selector's
+        features."""
+        self._obs = (x, z, float(mu))
+        self._iteration += 1
         nm = max(float(mu), 0.0) / (1.0 + max(float(mu), 0.0))
         self.current_cg_tol = min(self.cg_tol_max, max(self.cg_tol, min(self.forcing_factor * nm, self._ew_tolerance)))
 
@@ -347,6 +367,37 @@ class HipSpectralStableColumnPreconditioner:
                     scores = scores + self.residual_score_weight * max(s_sc, 1.0) * (impact / i_sc)
         return scores
 
+    def selector_features(self, x, z, mu, iteration):
+        """Raw learned-selector features of the candidates (ncand x 8, device-built)."""
+        out = np.empty((self.candidates.size, 8), dtype=np.float32)
+        self._call("psg_select_features", self._ctx, self._candidates32, self.candidates.size,
+                   np.ascontiguousarray(x, dtype=float), np.ascontiguousarray(z, dtype=float), float(mu),
+                   float(iteration), out)
+        return out
+
+    def _learned_scores(self):
+        """Network probabilities of the candidates, or None when not applicable."""
+        if not self._learned_active or self._obs is None or not self.candidates.size:
+            return None
+        x, z, mu = self._obs
+        logits = np.empty(self.candidates.size, dtype=np.float32)
+        self._call("psg_select_logits", self._ctx, self.selector.handle, self._candidates32, self.candidates.size,
+                   np.ascontiguousarray(x, dtype=float), np.ascontiguousarray(z, dtype=float), float(mu),
+                   float(self._iteration), logits)
+        if not np.isfinite(logits).all():
+            return None
+        return 1.0 / (1.0 + np.exp(-logits.astype(float)))
+
+    def _record_trajectory(self, weights, delta):
+        """Features now, labelled later with the rule's pick (TrajectoryRecorder)."""
+        if self._obs is None or not self.candidates.size:
+            return
+        scores = self._scores(weights, delta)
+        target = min(self.core_target, self.candidates.size)
+        pick = self.candidates[top_k(scores, target)]
+        x, z, mu = self._obs
+        self.recorder.record(self.candidates, np.sort(pick), self.selector_features(x, z, mu, self._iteration))
+
     def _refresh_core(self, weights, delta, stable):
         """Mandatory + retained + top-score fill (SAP:311-371, SSP:388-394)."""
         old = self.core.copy()
@@ -355,7 +406,12 @@ class HipSpectralStableColumnPreconditioner:
             self.core = self.candidates.copy()
             self.retained_fractions.append(1.0)
         else:
-            scores = self._scores(weights, delta)
+            scores = self._learned_scores() if self.selector is not None else None
+            self._learned_pending = scores is not None
+            if scores is not None:
+                self.learned_refreshes += 1
+            else:
+                scores = self._scores(weights, delta)
             target = min(self.core_target, self.candidates.size)
             picked = []
             mand_local = np.searchsorted(self.candidates, self.mandatory)
@@ -491,6 +547,8 @@ class HipSpectralStableColumnPreconditioner:
         theta = np.ascontiguousarray(theta, dtype=float)
         self._weights = 1.0 / theta
         self._call("psg_set_weights", self._ctx, theta, float(delta))
+        if self.recorder is not None:
+            self._record_trajectory(self._weights, float(delta))
         spectral_poor = self._last_condition >= self.condition_limit and self._last_cg >= max(10, int(0.8 * self.refresh_cg))
         poor = self._last_cg >= self.refresh_cg or spectral_poor or self._last_stagnated
         self._poor_streak = self._poor_streak + 1 if poor else 0
@@ -582,7 +640,21 @@ class HipSpectralStableColumnPreconditioner:
         rel = float(np.linalg.norm(rhs - matvec(sol))) / scale
         return sol[: self.n], -sol[self.n:], bool(np.isfinite(rel) and rel <= target), count[0], rel
 
+    def _judge_learned(self, res: KrylovResult):
+        """Fallback of the learned selector: a poor solve right after a learned
+        refresh counts a strike; `selector_patience` strikes in a row switch
+        the object back to the leverage rule."""
+        if not self._learned_pending:
+            return
+        self._learned_pending = False
+        poor = res.iterations >= self.refresh_cg or res.info != 0 or res.stagnated
+        self._learned_strikes = self._learned_strikes + 1 if poor else 0
+        if self._learned_strikes >= self.selector_patience:
+            self._learned_active = False
+            self.learned_fallbacks += 1
+
     def _record(self, res: KrylovResult):
+        self._judge_learned(res)
         self._last_cg = int(res.iterations)
         self._last_condition = float(res.condition_estimate)
         self._last_spectral_error = float(res.spectral_error_estimate)
@@ -650,6 +722,78 @@ class HipSpectralStableColumnPreconditioner:
         self.n_solve += 1
         self.t_solve += max(0.0, time.perf_counter() - started - (self.t_fact - fact_before))
         return dx, dy
+
+
+    # ------------------------------------------------------ two right-hand sides
+    def _pcg_pair(self, tols):
+        """Block PCG on the two device-resident right-hand sides."""
+        res = (_lib.PsgPcgResult * 2)()
+        cap = self.cg_maxit + 2
+        alphas, betas = np.empty(2 * self.cg_maxit), np.empty(2 * self.cg_maxit)
+        hist = np.empty(2 * cap)
+        self._call("psg_pcg2", self._ctx, np.ascontiguousarray(tols, dtype=float), self.cg_maxit,
+                   self.residual_replacement, self.residual_gap_tolerance, res, alphas, betas, hist)
+        pms = C.c_double(0.0)
+        self._call("psg_last_timings", self._ctx, C.byref(pms), None)
+        self.device_pcg_ms += pms.value
+        self.pcg_calls += 1
+        out = []
+        for q in range(2):
+            r = res[q]
+            a = alphas[q * self.cg_maxit: q * self.cg_maxit + r.n_alpha]
+            b = betas[q * self.cg_maxit: q * self.cg_maxit + r.n_beta]
+            lo, hi, cond = ritz_estimate(a, b)
+            h = hist[q * cap: q * cap + r.n_history]
+            wdw = self.stagnation_window
+            stagnated = h.size >= wdw and h[-1] > self.stagnation_reduction * h[-wdw]
+            out.append(KrylovResult(None, int(r.info), int(r.iterations), float(r.relative_residual), float(cond),
+                                    float(max(abs(lo - 1.0), abs(hi - 1.0))), None, int(r.replacements),
+                                    int(r.residual_checks), bool(stagnated)))
+        return out
+
+    def solve_pair(self, r1a, r2a, r1b, r2b):
+        """Two independent Newton systems with the same matrix, solved together.
+
+        Both right-hand sides run one block PCG (independent recurrences,
+        each with the tolerance and the acceptance rule of ``solve``), so every
+        read of A and of the factor serves both.  A column that fails is
+        solved again by ``solve``, with its retry and fallback ladder
+        (SSP:651-752).  Returns ((dx_a, dy_a), (dx_b, dy_b)).
+        """
+        started = time.perf_counter()
+        fact_before = self.t_fact
+        r1 = np.ascontiguousarray(np.stack([r1a, r1b], axis=1), dtype=float)
+        r2 = np.ascontiguousarray(np.stack([r2a, r2b], axis=1), dtype=float)
+        self._call("psg_solve_begin2", self._ctx, r1, r2)
+        tols = [self._linear_tolerance(), self._linear_tolerance()]
+        results = self._pcg_pair(tols)
+        ok = []
+        for q, res in enumerate(results):
+            tol = tols[q]
+            failed = res.info != 0 or not np.isfinite(res.relative_residual) or res.relative_residual > tol
+            if failed and res.relative_residual <= max(2.0 * tol, self.fail_tol) and np.isfinite(res.relative_residual):
+                failed = False
+            ok.append(not failed)
+        dx, dy = np.empty((self.n, 2)), np.empty((self.m, 2))
+        self._call("psg_solve_finish2", self._ctx, dx, dy)
+        out = []
+        for q, res in enumerate(results):
+            if ok[q]:
+                self.cg_tolerances.append(tols[q])
+                self.cg_iterations.append(res.iterations)
+                self._record(res)
+                if res.info > 0:
+                    self.krylov_iteration_limits += 1
+                self.n_solve += 1
+                out.append((np.ascontiguousarray(dx[:, q]), np.ascontiguousarray(dy[:, q])))
+            else:
+                self.pair_fallbacks += 1
+                self.t_solve += max(0.0, time.perf_counter() - started - (self.t_fact - fact_before))
+                out.append(self.solve((r1a, r1b)[q], (r2a, r2b)[q]))
+                started, fact_before = time.perf_counter(), self.t_fact
+        self.pair_solves += 1
+        self.t_solve += max(0.0, time.perf_counter() - started - (self.t_fact - fact_before))
+        return out[0], out[1]
 
 
 def solve_standard(lp, precision="fp64", **kwargs):

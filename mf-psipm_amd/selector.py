@@ -1,0 +1,206 @@
+"""Learned column selection for the active-column preconditioner.
+
+The reference picks the preconditioner's core columns by a leverage rule
+(psipm_stable_preconditioner.py:293-371).  ``LearnedColumnSelector`` adds a
+per-column MLP (8 -> 64 -> 64 -> 1, ReLU) that predicts which candidate
+columns the rule will put in the core at the next interior-point iteration,
+from each column's trajectory: log10 theta_j at the last three
+factorisations, log10 x_j, log10 z_j, log10 ||a_j||_2, log10 mu and the
+iteration index.  Inference (BF16 MFMA, FP32 accumulation), training (FP32
+forward/backward, Adam) and the feature construction are HIP kernels of
+libpsipm_hip.so; this module holds the parameters, the trajectory recorder
+and the training driver.
+
+``TrajectoryRecorder`` is attached to the GPU preconditioner object; on every
+factorisation it records the features of a sample of candidate columns and,
+one iteration later, labels them with the rule's core of that iteration.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+
+N_IN = 8
+
+
+class LearnedColumnSelector:
+    """Device-resident MLP scoring candidate columns (``psg_mlp`` handle)."""
+
+    def __init__(self, params: Optional[np.ndarray] = None, norm: Optional[np.ndarray] = None, seed: int = 0):
+        self._lib = _lib.load()
+        self.n_params = int(self._lib.psg_mlp_num_params())
+        h = C.c_void_p()
+        _lib.check("psg_mlp_create", self._lib.psg_mlp_create(C.byref(h)))
+        self.handle = h
+        if params is None:
+            params = he_init(self.n_params, seed)
+        if norm is None:
+            norm = np.concatenate([np.zeros(N_IN), np.ones(N_IN)])
+        self.set_params(params, norm)
+
+    # ----------------------------------------------------------------- params
+    def set_params(self, params, norm):
+        self.norm = np.ascontiguousarray(norm, dtype=np.float32)
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != self.n_params or self.norm.size != 2 * N_IN:
+            raise ValueError("parameter or normalisation vector has the wrong size")
+        _lib.check("psg_mlp_set_params", self._lib.psg_mlp_set_params(self.handle, p, self.norm))
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n_params, dtype=np.float32)
+        _lib.check("psg_mlp_get_params", self._lib.psg_mlp_get_params(self.handle, out))
+        return out
+
+    def save(self, path):
+        np.savez(path, params=self.params(), norm=self.norm)
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path, allow_pickle=False) as d:
+            return cls(params=d["params"], norm=d["norm"])
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.psg_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------- compute
+    def forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.empty(X.shape[0], dtype=np.float32)
+        _lib.check("psg_mlp_forward", self._lib.psg_mlp_forward(self.handle, X, X.shape[0], out))
+        return out
+
+    def loss_grad(self, X, y, pos_weight=1.0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.empty(self.n_params, dtype=np.float32)
+        loss = C.c_float()
+        _lib.check("psg_mlp_loss_grad", self._lib.psg_mlp_loss_grad(self.handle, X, y, X.shape[0], float(pos_weight),
+                                                                     C.byref(loss), g))
+        return float(loss.value), g
+
+    def fit(self, X, y, steps=2000, batch=4096, lr=2e-3, pos_weight=None, seed=0, normalise=True):
+        """Train on (features, labels) with Adam; returns the minibatch losses."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        if normalise:
+            mean = X.mean(axis=0)
+            std = X.std(axis=0)
+            istd = np.where(std > 1e-6, 1.0 / np.maximum(std, 1e-6), 1.0)
+            self.set_params(self.params(), np.concatenate([mean, istd]))
+        if pos_weight is None:
+            pos = float(y.mean())
+            pos_weight = (1.0 - pos) / max(pos, 1e-6) if 0.0 < pos < 1.0 else 1.0
+        losses = np.empty(steps, dtype=np.float32)
+        _lib.check("psg_mlp_train", self._lib.psg_mlp_train(
+            self.handle, X, y, X.shape[0], int(batch), int(steps), float(lr), float(pos_weight), int(seed), losses))
+        return losses
+
+
+def he_init(n_params: int, seed: int = 0) -> np.ndarray:
+    """He-uniform weights, zero biases, in the parameter layout of the library."""
+    rng = np.random.default_rng(seed)
+    t = np.zeros(n_params, dtype=np.float32)
+    o_b1, o_w2 = 8 * 64, 8 * 64 + 64
+    o_b2, o_w3 = o_w2 + 64 * 64, o_w2 + 64 * 64 + 64
+    t[:o_b1] = rng.uniform(-np.sqrt(6 / 8), np.sqrt(6 / 8), o_b1)
+    t[o_w2:o_b2] = rng.uniform(-np.sqrt(6 / 64), np.sqrt(6 / 64), 64 * 64)
+    t[o_w3:o_w3 + 64] = rng.uniform(-np.sqrt(3 / 64), np.sqrt(3 / 64), 64)
+    return t
+
+
+class TrajectoryRecorder:
+    """Collects (features, label) pairs along an interior-point run.
+
+    At each factorisation the preconditioner object calls ``record`` with the
+    rule's core of that iteration and the features of all candidates.  The
+    sample recorded at iteration k is labelled with the core of iteration
+    k + 1 ("next-iteration active columns").  Per iteration at most
+    ``per_iter`` columns are kept: every column of either core plus a
+    uniform sample of the rest.
+    """
+
+    def __init__(self, per_iter: int = 20000, seed: int = 0):
+        self.per_iter = int(per_iter)
+        self.rng = np.random.default_rng(seed)
+        self._pending = None
+        self.X, self.y = [], []
+
+    def record(self, candidates: np.ndarray, core: np.ndarray, features: np.ndarray):
+        in_core = np.isin(candidates, core, assume_unique=True)
+        if self._pending is not None:
+            rows, feats = self._pending
+            self.X.append(feats)
+            self.y.append(in_core[rows].astype(np.float32))
+        # rows to keep for the next label: current core members and a sample
+        core_rows = np.flatnonzero(in_core)
+        rest = np.flatnonzero(~in_core)
+        k = max(0, min(rest.size, self.per_iter - core_rows.size))
+        pick = np.concatenate([core_rows, self.rng.choice(rest, size=k, replace=False) if k else rest[:0]])
+        self._pending = (pick, features[pick].copy())
+
+    def end_run(self):
+        """Drop the unlabelled sample of the last iteration of a run."""
+        self._pending = None
+
+    def dataset(self):
+        if not self.X:
+            return np.empty((0, N_IN), np.float32), np.empty(0, np.float32)
+        return np.concatenate(self.X).astype(np.float32), np.concatenate(self.y).astype(np.float32)
+
+
+# --------------------------------------------------------------------------
+# training and evaluation drivers
+# --------------------------------------------------------------------------
+def setcover_problem(m: int, n: int, seed: int):
+    """Set-covering LP min c'x : Ax >= 1, x >= 0 through the solver pipeline
+    (canonical form with surplus columns, equilibration, internal tolerance)."""
+    from . import generators, lp as lpmod
+
+    raw = generators.set_covering(m, n, seed)
+    lp = lpmod.to_standard_form(raw)
+    lpmod.scale_problem(lp)
+    return raw, lp, lpmod.scaled_tolerance(lp, 1e-8)
+
+
+def solve_with(lp, tol, selector=None, recorder=None, precision="fp64", maxit=160, **opts):
+    """One PS-IPM solve on the GPU object; returns (Solution, solver counters)."""
+    import time
+
+    from .ipm import psipm_solve
+    from .preconditioner import HipSpectralStableColumnPreconditioner
+
+    kw = dict(cg_tol=1e-8, cg_tol_max=1e-8, structural_backbone=False)
+    kw.update(opts)
+    solver = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, precision=precision,
+                                                   selector=selector, recorder=recorder, **kw)
+    t0 = time.perf_counter()
+    sol = psipm_solve(lp, solver, tol=tol, compl="gap", maxit=maxit)
+    wall = time.perf_counter() - t0
+    stats = dict(seconds=wall, status=sol.status, gap=float(sol.gap), pcg=int(sol.pcg_iterations),
+                 factorizations=int(sol.preconditioner_factorizations), refreshes=int(solver.refreshes),
+                 retries=int(solver.pcg_retries), learned_refreshes=int(solver.learned_refreshes),
+                 learned_fallbacks=int(solver.learned_fallbacks), t_fact=float(sol.t_fact),
+                 t_solve=float(sol.t_solve), obj=float(sol.obj))
+    solver.close()
+    return sol, stats
+
+
+def collect_trajectories(problems, per_iter=20000, seed=0, **opts):
+    """Run the rule-based solver on (lp, tol) pairs and record (features, labels)."""
+    rec = TrajectoryRecorder(per_iter=per_iter, seed=seed)
+    for lp, tol in problems:
+        solve_with(lp, tol, recorder=rec, **opts)
+        rec.end_run()
+    return rec.dataset()

@@ -1,0 +1,117 @@
+"""GPU parity of the learned selector's kernels against the oracle network.
+
+Tolerances: inference runs BF16 operands with FP32 accumulation, compared
+with the oracle's BF16-rounded forward (accumulation order only, so
+|dz| <= 2e-3 (1 + |z|)) and with the FP32 forward (|dz| <= 5e-2 (1 + |z|));
+the FP32 training gradient against the float64 oracle gradient to 1e-4 of
+its norm; one Adam step against the oracle update to 1e-5 absolute.
+"""
+import numpy as np
+import pytest
+
+import mf_psipm_amd as P
+from mf_psipm_amd.selector import LearnedColumnSelector
+from oracle import mlp_cpu as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def _data(seed, B):
+    rng = np.random.default_rng(seed)
+    X = (rng.standard_normal((B, 8)) * 2.0 + 1.0).astype(np.float32)
+    y = (rng.uniform(size=B) < 0.25).astype(np.float32)
+    params = M.init_params(seed)
+    params[M.OFF_B1:M.OFF_W2] = rng.uniform(-0.1, 0.1, 64)
+    params[M.OFF_B2:M.OFF_W3] = rng.uniform(-0.1, 0.1, 64)
+    norm = np.concatenate([np.full(8, 1.0), np.full(8, 0.5)]).astype(np.float32)
+    return X, y, params, norm
+
+
+def _normed(X, norm):
+    return ((X - norm[:8]) * norm[8:]).astype(np.float32)
+
+
+@pytest.mark.parametrize("B", [1, 15, 16, 1000, 100003])
+def test_forward_bf16_matches_oracle(gpu, B):
+    _say(f"mlp forward B={B}")
+    X, _, params, norm = _data(B, B)
+    net = LearnedColumnSelector(params, norm)
+    z = net.forward(X)
+    zb = M.forward_bf16(params, _normed(X, norm))
+    z32, _ = M.forward(params, _normed(X, norm))
+    assert np.all(np.abs(z - zb) <= 2e-3 * (1.0 + np.abs(zb)))
+    assert np.all(np.abs(z - z32) <= 5e-2 * (1.0 + np.abs(z32)))
+    net.close()
+
+
+@pytest.mark.parametrize("B,pw", [(7, 1.0), (128, 3.0), (5000, 2.5)])
+def test_loss_grad_matches_oracle(gpu, B, pw):
+    _say(f"mlp loss/grad B={B} pw={pw}")
+    X, y, params, norm = _data(10 + B, B)
+    net = LearnedColumnSelector(params, norm)
+    loss, g = net.loss_grad(X, y, pw)
+    L, G = M.loss_and_grad(params.astype(np.float64), _normed(X, norm).astype(np.float64), y, pw)
+    assert abs(loss - L) <= 1e-5 * (1.0 + abs(L))
+    assert np.linalg.norm(g - G) <= 1e-4 * np.linalg.norm(G)
+    net.close()
+
+
+def test_adam_step_matches_oracle(gpu):
+    _say("mlp one Adam step, full batch")
+    B = 512
+    X, y, params, norm = _data(77, B)
+    net = LearnedColumnSelector(params, norm)
+    losses = net.fit(X, y, steps=1, batch=B, lr=1e-3, pos_weight=2.0, normalise=False)
+    _, G = M.loss_and_grad(params.astype(np.float64), _normed(X, norm).astype(np.float64), y, 2.0)
+    ref, _, _ = M.adam_step(params.astype(np.float64), G, np.zeros_like(G), np.zeros_like(G), 1, lr=1e-3)
+    got = net.params()
+    # components whose gradient is far from zero move by ~lr; near-zero
+    # gradients can flip the FP32 sign, so compare where |G| is resolvable
+    ok = np.abs(G) > 1e-4 * np.abs(G).max()
+    assert np.max(np.abs(got[ok] - ref[ok])) <= 1e-5
+    assert np.isfinite(losses).all()
+    net.close()
+
+
+def test_training_learns_a_separable_rule(gpu):
+    _say("mlp training on a synthetic rule")
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((40000, 8)).astype(np.float32)
+    y = ((X[:, 0] - X[:, 1] + 0.5 * X[:, 3] * X[:, 4]) > 0.3).astype(np.float32)
+    net = LearnedColumnSelector(seed=1)
+    losses = net.fit(X, y, steps=600, batch=2048, lr=3e-3, seed=2)
+    assert losses[-20:].mean() < 0.5 * losses[:20].mean()
+    acc = np.mean((net.forward(X) > 0) == (y > 0.5))
+    assert acc > 0.95
+    # inference agrees with the oracle on the trained parameters
+    zb = M.forward_bf16(net.params(), _normed(X[:2000], net.norm))
+    assert np.all(np.abs(net.forward(X[:2000]) - zb) <= 2e-3 * (1.0 + np.abs(zb)))
+    net.close()
+
+
+def test_select_features_match_numpy(gpu):
+    _say("selector features on a 300 x 900 LP")
+    from mf_psipm_amd.preconditioner import HipSpectralStableColumnPreconditioner
+    raw, *_ = P.generators.random_lp(300, 900, 4, 3)
+    lp = P.lp.to_standard_form(raw)
+    A = lp.A
+    obj = HipSpectralStableColumnPreconditioner(A, protected=~lp.pos)
+    rng = np.random.default_rng(0)
+    thetas = [10.0 ** rng.uniform(-3, 3, A.shape[1]) for _ in range(4)]
+    x = rng.uniform(1e-6, 2.0, A.shape[1])
+    z = rng.uniform(1e-6, 2.0, A.shape[1])
+    cand = obj._candidates32
+    for k, th in enumerate(thetas):
+        obj._call("psg_set_weights", obj._ctx, th, 1e-6)
+        f = obj.selector_features(x, z, 0.37, k)
+        h1 = thetas[max(0, k - 1)]
+        h2 = thetas[max(0, k - 2)]
+        ref = np.stack([np.log10(th[cand]), np.log10(h1[cand]), np.log10(h2[cand]), np.log10(x[cand]),
+                        np.log10(z[cand]), np.log10(np.sqrt(np.asarray(A.multiply(A).sum(axis=0)).ravel()[cand])),
+                        np.full(cand.size, np.log10(0.37)), np.full(cand.size, float(k))], axis=1)
+        assert np.allclose(f, ref, rtol=1e-6, atol=1e-6)
+    obj.close()
