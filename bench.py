@@ -86,12 +86,27 @@ def gpu_leg(lp, args, precision):
     iters = max(1, res.iterations)
     kt = np.zeros(4)
     solver._call("psg_time_kernels", solver._ctx, 20, kt)
+    # two right-hand sides in one block PCG (every read of A and L serves both)
+    pair = None
+    if not args.no_pair:
+        R1 = np.zeros((n, 2))
+        R2 = np.ascontiguousarray(np.stack([r2, rng.standard_normal(m)], axis=1))
+        solver._call("psg_solve_begin2", solver._ctx, R1, R2)
+        tols = np.array([1e-30, 1e-30])
+        res2 = (_lib.PsgPcgResult * 2)()
+        solver._call("psg_pcg2", solver._ctx, tols, max(1, args.warmup), 0, 0.1, res2, None, None, None)
+        t0 = time.perf_counter()
+        solver._call("psg_pcg2", solver._ctx, tols, args.steps, 0, 0.1, res2, None, None, None)
+        wall2 = time.perf_counter() - t0
+        col_its = res2[0].iterations + res2[1].iterations
+        pair = dict(column_iterations=int(col_its), wall_s=wall2, it_per_s=col_its / wall2,
+                    info=[int(res2[0].info), int(res2[1].info)])
     out = dict(
         precision=precision, iterations=int(res.iterations), info=int(res.info), wall_s=wall,
         device_ms=dev.value, it_per_s=iters / wall, factor_s=factor_wall,
         factor_device_ms=solver.device_factor_ms, core=int(solver.core.size),
         nnz_L=int(solver.nnz_factor), plan=dict(solver._plan.info), hv_ms=kt[0], hv_loop_ms=kt[1],
-        apply_ms=kt[2], dot_ms=kt[3], theta=theta, r2=r2, core_cols=solver.core.copy(),
+        apply_ms=kt[2], dot_ms=kt[3], theta=theta, r2=r2, core_cols=solver.core.copy(), pair=pair,
     )
     solver.close()
     return out
@@ -141,27 +156,84 @@ def cpu_leg(lp, theta, r2, seconds, core_factor):
                 nnz_L=int(cpu._factor.nnz))
 
 
-def time_to_gap(m, n, k, seed, precision):
+def time_to_gap(m, n, k, seed, precision, pc=3, selector=None, problem=None, **opts):
     """Full PS-IPM solve to 1e-8 relative gap on the GPU path (setup excluded)."""
     import mf_psipm_amd as P
     from mf_psipm_amd.preconditioner import HipSpectralStableColumnPreconditioner
 
-    raw, *_ = P.generators.random_lp(m, n, k, seed)
-    lp = P.lp.to_standard_form(raw)
-    P.lp.scale_problem(lp)
-    tol = P.lp.scaled_tolerance(lp, 1e-8)
+    if problem is None:
+        raw, *_ = P.generators.random_lp(m, n, k, seed)
+        lp = P.lp.to_standard_form(raw)
+        P.lp.scale_problem(lp)
+        tol = P.lp.scaled_tolerance(lp, 1e-8)
+    else:
+        raw, lp, tol = problem
     solver = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, precision=precision,
-                                                   cg_tol=1e-8, cg_tol_max=1e-8)
+                                                   cg_tol=1e-8, cg_tol_max=1e-8, selector=selector, **opts)
     t0 = time.perf_counter()
-    sol = P.ipm.psipm_solve(lp, solver, tol=tol, compl="gap", maxit=160)
+    sol = P.ipm.psipm_solve(lp, solver, tol=tol, compl="gap", maxit=160, pc=pc)
     wall = time.perf_counter() - t0
     info = P.lp.recover(lp, raw, sol.x, sol.y, sol.z, sol.w)
     out = dict(seconds=round(wall, 3), status=sol.status, gap=float(sol.gap), pinf=info["pinf"], dinf=info["dinf"],
                objective=info["obj"], pcg_iterations=int(sol.pcg_iterations),
                pcg_it_per_s=round(sol.pcg_iterations / max(1e-9, sol.t_solve), 1),
                factorizations=int(sol.preconditioner_factorizations), t_fact=round(sol.t_fact, 3),
-               t_solve=round(sol.t_solve, 3))
+               t_solve=round(sol.t_solve, 3), pair_solves=int(solver.pair_solves),
+               learned_refreshes=int(solver.learned_refreshes), learned_fallbacks=int(solver.learned_fallbacks))
     solver.close()
+    return out
+
+
+def batch_leg(count, seed0, precision, core_factor, tol, maxit, graph_iters, sequential):
+    """Independent random LPs (m ~ U{5000..50000}, n = 3m, 6 nnz/col) solved
+    as one batch: one launch of each hot-path kernel per PCG step for all LPs,
+    replayed as a HIP graph; against the same LPs solved one after another."""
+    import numpy as np
+
+    import mf_psipm_amd as P
+    from mf_psipm_amd.batch import BatchedNormalEquations
+
+    sizes = P.generators.batch_sizes(count, seed0)
+    mats = []
+    for i, mi in enumerate(sizes):
+        raw, *_ = P.generators.random_lp(mi, 3 * mi, 6, seed0 + i)
+        lp = P.lp.to_standard_form(raw)
+        P.lp.scale_problem(lp)
+        mats.append(lp.A)
+    rng = np.random.default_rng(seed0)
+    thetas = [10.0 ** rng.uniform(-4.0, 4.0, A.shape[1]) for A in mats]
+    rhs = [rng.standard_normal(A.shape[0]) for A in mats]
+    B = BatchedNormalEquations(mats, precision=precision, core_factor=core_factor)
+    t0 = time.perf_counter()
+    B.factorize(thetas, 1e-6)
+    fact_s = time.perf_counter() - t0
+    B.solve(rhs, tol=tol, maxit=maxit, graph_iters=graph_iters)  # warm (graph capture)
+    t0 = time.perf_counter()
+    dys, st = B.solve(rhs, tol=tol, maxit=maxit, graph_iters=graph_iters)
+    batched_s = time.perf_counter() - t0
+    errs = []
+    for A, th, b, dy in zip(mats, thetas, rhs, dys):
+        Hdy = A @ ((1.0 / th) * (A.T @ dy)) + 1e-6 * dy
+        errs.append(float(np.linalg.norm(b - Hdy) / max(1.0, np.linalg.norm(b))))
+    out = dict(lps=count, rows=int(B.m), cols=int(B.n), nnz=int(B.A.nnz), core=int(B.core.size),
+               nnz_L=int(B.nnz_factor), factor_s=round(fact_s, 3), batched_s=round(batched_s, 4),
+               device_ms=round(st["device_ms"], 3), replays=int(st["replays"]),
+               iterations_total=int(st["iterations"].sum()), iterations_max=int(st["iterations"].max()),
+               converged=int((st["info"] == 0).sum()), fp64_residual_max=max(errs),
+               fp64_residual_per_lp=[float(f"{e:.3e}") for e in errs])
+    B.close()
+    if sequential:
+        seq = 0.0
+        for A, th, b in zip(mats, thetas, rhs):
+            Bi = BatchedNormalEquations([A], precision=precision, core_factor=core_factor)
+            Bi.factorize([th], 1e-6)
+            Bi.solve([b], tol=tol, maxit=maxit, graph_iters=0)
+            t0 = time.perf_counter()
+            Bi.solve([b], tol=tol, maxit=maxit, graph_iters=0)
+            seq += time.perf_counter() - t0
+            Bi.close()
+        out["sequential_s"] = round(seq, 4)
+        out["speedup"] = round(seq / batched_s, 2)
     return out
 
 
@@ -181,6 +253,12 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-gap", action="store_true")
     ap.add_argument("--cpu-core-factor", type=float, default=0.05)
+    ap.add_argument("--no-pair", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--batch-core-factor", type=float, default=0.1)
+    ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--selector", default=os.path.join(ROOT, "mf-psipm_amd", "models", "selector_setcover.npz"))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -208,9 +286,27 @@ def main():
             f"factor {cpu['factor_s']:.1f}s)")
     gaps = {}
     if not args.no_gap:
-        for prec in ("fp64", "mixed"):
-            gaps[prec] = time_to_gap(2000, 6000, 5, 0, prec)
-            log(f"time-to-1e-8-gap 2000x6000 {prec}: {gaps[prec]}")
+        for name, prec, pc in (("fp64", "fp64", 3), ("mixed", "mixed", 3), ("mixed_block", "mixed", 4)):
+            gaps[name] = time_to_gap(2000, 6000, 5, 0, prec, pc=pc)
+            log(f"time-to-1e-8-gap 2000x6000 {name}: {gaps[name]}")
+    learned = None
+    if not args.no_gap and os.path.exists(args.selector):
+        from mf_psipm_amd.selector import LearnedColumnSelector, setcover_problem
+
+        sel = LearnedColumnSelector.load(args.selector)
+        prob = setcover_problem(2000, 100000, 0)
+        learned = {}
+        for name, prec, pc, use in (("plain_fp64", "fp64", 3, False), ("learned_mixed_block", "mixed", 4, True)):
+            learned[name] = time_to_gap(0, 0, 0, 0, prec, pc=pc, selector=sel if use else None, problem=prob,
+                                        structural_backbone=False)
+            log(f"time-to-1e-8-gap set cover 2000x100000 {name}: {learned[name]}")
+        learned["speedup"] = round(learned["plain_fp64"]["seconds"] / learned["learned_mixed_block"]["seconds"], 3)
+        sel.close()
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(args.batch_count, 100, "fp64", args.batch_core_factor, 1e-8, 1000, 8,
+                          not args.no_sequential)
+        log(f"batch: {batch}")
 
     bytes_hv = hv_bytes(m, n, nnz, 8)
     achieved = bytes_hv / (g64["hv_ms"] * 1e-3) / 1e9
@@ -249,6 +345,11 @@ def main():
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": int(bytes_hv),
                         "ms_per_launch": round(g64["hv_ms"], 4)},
         "time_to_gap_2000x6000": gaps or None,
+        "time_to_gap_setcover_2000x100000": learned,
+        "pair_column_it_per_s": None if g64["pair"] is None else round(g64["pair"]["it_per_s"], 3),
+        "pair_column_it_per_s_mixed": None if mixed is None or mixed["pair"] is None
+        else round(mixed["pair"]["it_per_s"], 3),
+        "batch_64_lps": batch,
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["it_per_s"], 4), "unit": "PCG it/s", "cores": 1, "kind": "port",
             "sample": f"{cpu['iterations']} PCG iterations (~{args.cpu_seconds:.0f}s) of the FP64 CPU "

@@ -150,6 +150,40 @@ int psg_solve_finish2(psg_ctx* ctx, double* dx, double* dy);
 int psg_apply_H2(psg_ctx* ctx, const double* v, double* out);
 int psg_apply_P2(psg_ctx* ctx, const double* r, double* z);
 
+/* ---- batched LPs -------------------------------------------------------------
+ * No counterpart in the reference (one LP per process, batch_test_*.py).
+ * Independent LPs are stacked block-diagonally into one context
+ * (psg_create on diag(A_1, ..., A_k)); row_off (k + 1 entries, 0 .. m) marks
+ * each LP's rows.  One factor of the block-diagonal active-column matrix
+ * (psg_factor) serves all LPs.  psg_batch_pcg runs SSP:469-581's recurrences
+ * with per-LP scalars, guards (info -1, -2, -3, -6, -7) and tolerances, one
+ * launch of each hot-path kernel per step for all LPs; an LP is frozen once
+ * its recurrence residual meets tol[lp].  With graph_iters > 0 the step is
+ * captured once as a HIP graph of graph_iters iterations and replayed.
+ * Outputs: x (m), per-LP iterations, info, explicit relative residual. */
+typedef struct psg_batch psg_batch;
+int psg_batch_create(psg_ctx* ctx, int32_t nlp, const int64_t* row_off, psg_batch** out);
+int psg_batch_destroy(psg_batch* batch);
+int psg_batch_pcg(psg_batch* batch, const double* rhs, const double* tol, int32_t maxit, int32_t graph_iters,
+                  double* x, int32_t* iterations, int32_t* info, double* relative_residual);
+/* device time (ms, HIP events) of the last psg_batch_pcg and its replay count */
+int psg_batch_last(psg_batch* batch, double* device_ms, int32_t* replays);
+
+/* ---- interior-point vector kernels -------------------------------------------
+ * out = A in (trans 0, in: n, out: m) or A^T in (trans 1, in: m, out: n):
+ * the products of the residuals (PS:1984-1993, 2561-2562) */
+int psg_spmv(psg_ctx* ctx, int32_t trans, const double* in, double* out);
+/* keep the iterate (x, z, s, w: n each) and, when given, the masks of the
+ * nonnegative and boxed columns on the device for the step kernels; called
+ * once per IPM iteration from the observe hook (SSP:252-258) */
+int psg_ipm_observe(psg_ctx* ctx, const double* x, const double* z, const double* s, const double* w,
+                    const uint8_t* pos, const uint8_t* box);
+/* *ratio = min(1, step to the boundary): dual 0 -> primal_step's ratio
+ * min(step_to_boundary(x, d1, pos), step_to_boundary(s, -d1, box)); dual 1 ->
+ * dual_step's min(step_to_boundary(z, d1, pos), step_to_boundary(w, d2, box))
+ * (PS:1831-1837, 2101-2106; d2 may be NULL when no column is boxed) */
+int psg_ipm_step(psg_ctx* ctx, int32_t dual, const double* d1, const double* d2, double* ratio);
+
 /* ---- timing ---------------------------------------------------------------- */
 /* Device time of the hot-path pieces, measured with HIP events on the
  * library's own stream: out[0] = H v (both gathers, FP64 values),

@@ -97,6 +97,13 @@ SIGNATURES = [
     ("psg_solve_finish2", C.c_int, [C.c_void_p, _f64p, _f64p]),
     ("psg_apply_H2", C.c_int, [C.c_void_p, _f64p, _f64p]),
     ("psg_apply_P2", C.c_int, [C.c_void_p, _f64p, _f64p]),
+    ("psg_batch_create", C.c_int, [C.c_void_p, C.c_int32, _i64p, C.POINTER(C.c_void_p)]),
+    ("psg_batch_destroy", C.c_int, [C.c_void_p]),
+    ("psg_batch_pcg", C.c_int, [C.c_void_p, _f64p, _f64p, C.c_int32, C.c_int32, _f64p, _i32p, _i32p, _f64p]),
+    ("psg_batch_last", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    ("psg_spmv", C.c_int, [C.c_void_p, C.c_int32, _f64p, _f64p]),
+    ("psg_ipm_observe", C.c_int, [C.c_void_p, _f64p, _f64p, _f64p, _f64p, _opt(_u8p), _opt(_u8p)]),
+    ("psg_ipm_step", C.c_int, [C.c_void_p, C.c_int32, _f64p, _opt(_f64p), C.POINTER(C.c_double)]),
     ("psg_mlp_create", C.c_int, [C.POINTER(C.c_void_p)]),
     ("psg_mlp_destroy", C.c_int, [C.c_void_p]),
     ("psg_mlp_num_params", C.c_int, []),

@@ -925,6 +925,11 @@ __global__ void k_mul2(int n, const double* __restrict__ w, const double* __rest
   }
 }
 
+// a device scalar set from a kernel argument (no host buffer to keep alive)
+__global__ void k_set_scalar(double* __restrict__ dst, double v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dst = v;
+}
+
 // dst column q <- src column q
 __global__ void k_copy_col2(int m, int q, const double* __restrict__ src, double* __restrict__ dst) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
@@ -997,7 +1002,7 @@ k_pcg_step2(int m, const double* __restrict__ hv, const double* __restrict__ rz0
 // beta = rz_new / rz, p = z + beta p; reset (after a residual replacement)
 // -> rz_new with the guard of SSP:528-531 (info -5), p = z; frozen -> none.
 __global__ void __launch_bounds__(kBlock)
-k_pcg_direction2(int m, const double* __restrict__ partC, int nbC, const int* __restrict__ mode,
+k_pcg_direction2(int m, const double* __restrict__ partC, int nbC, int mode0, int mode1,
                  const double* __restrict__ rz0, const double* __restrict__ rz1, double* __restrict__ rzn0,
                  double* __restrict__ rzn1, const double* __restrict__ z, double* __restrict__ p,
                  double* __restrict__ scal, double* __restrict__ be0, double* __restrict__ be1) {
@@ -1008,7 +1013,7 @@ k_pcg_direction2(int m, const double* __restrict__ partC, int nbC, const int* __
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     double* h = scal + c * kScalarHeader;
-    md[c] = (h[kStop] != 0.0) ? kDirFrozen : mode[c];
+    md[c] = (h[kStop] != 0.0) ? kDirFrozen : (c == 0 ? mode0 : mode1);
     if (md[c] == kDirFrozen) continue;
     const double rz_new = tot[c];
     int fail = 0;
@@ -1058,7 +1063,6 @@ static Block2& block2(psg_ctx* c) {
       b->partC.alloc(2 * kRedBlocks);
       b->hvdots.alloc(8);
       b->scal.alloc(2 * kScalarHeader + 2 * kRedBlocks);
-      b->mode.alloc(2);
       HIP_CHECK(hipHostMalloc((void**)&b->hscal, (2 * kScalarHeader + 2 * kRedBlocks) * sizeof(double)));
     } catch (...) {
       delete b;
@@ -1162,7 +1166,8 @@ int psg_pcg2(psg_ctx* c, const double* tol, int32_t maxit, int32_t replacement, 
     bool done[2] = {false, false};
     for (int q = 0; q < 2; ++q) {
       const double rz = tmp[q];
-      HIP_CHECK(hipMemcpyAsync(rzs[q], &rz, sizeof(double), hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, rzs[q], rz);
+      HIP_LAUNCH_CHECK();
       info[q] = maxit;
       if (hist[q].back() <= tolq[q]) info[q] = 0;
       else if (!std::isfinite(rz) || rz <= 0.0) info[q] = -1;
@@ -1171,8 +1176,8 @@ int psg_pcg2(psg_ctx* c, const double* tol, int32_t maxit, int32_t replacement, 
     HIP_CHECK(hipMemcpyAsync(b.p.p, b.z.p, 2 * (size_t)m * sizeof(double), hipMemcpyDeviceToDevice, s));
     const int grid_m = grid_for(m, kBlock, kRedBlocks);
     auto freeze = [&](int q) {
-      const double one = 1.0;
-      HIP_CHECK(hipMemcpyAsync(b.scal.p + q * kScalarHeader + kStop, &one, sizeof(double), hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, b.scal.p + q * kScalarHeader + kStop, 1.0);
+      HIP_LAUNCH_CHECK();
       done[q] = true;
     };
     for (int q = 0; q < 2; ++q)
@@ -1240,10 +1245,9 @@ int psg_pcg2(psg_ctx* c, const double* tol, int32_t maxit, int32_t replacement, 
         }
       }
       if (done[0] && done[1]) break;
-      HIP_CHECK(hipMemcpyAsync(b.mode.p, mode, 2 * sizeof(int), hipMemcpyHostToDevice, s));
       factor_apply2(c, b.r.p, b.z.p);
       const int nbC = device_dot2(c, b.r.p, b.z.p, b.partC.p);
-      hipLaunchKernelGGL(k_pcg_direction2, dim3(grid_m), dim3(kBlock), 0, s, m, b.partC.p, nbC, b.mode.p,
+      hipLaunchKernelGGL(k_pcg_direction2, dim3(grid_m), dim3(kBlock), 0, s, m, b.partC.p, nbC, mode[0], mode[1],
                          rzs[0] + rz_slot[0], rzs[1] + rz_slot[1], rzs[0] + it[0], rzs[1] + it[1], b.z.p, b.p.p,
                          b.scal.p, bes[0], bes[1]);
       HIP_LAUNCH_CHECK();
@@ -1333,6 +1337,463 @@ int psg_apply_P2(psg_ctx* c, const double* r, double* z) {
     HIP_CHECK(hipMemcpyAsync(z, b.q.p, 2 * (size_t)c->m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     if (factor_solve_error(c)) throw Error(PSG_ERR_INTERNAL, "triangular solve dependency wait timed out");
+    return PSG_OK;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched PCG: many independent LPs in one context.  Their matrices are
+// stacked block-diagonally (A = diag(A_1, ..., A_k)), so one launch of each
+// hot-path kernel -- both SpMV gathers, the triangular solves of the
+// block-diagonal factor (a forest of elimination trees) -- serves every LP,
+// whatever its size.  The PCG scalars are per LP: the vector kernels run a
+// 2-D grid (chunk, LP) and reduce per LP in a fixed order; an LP whose
+// recurrence residual meets its tolerance is frozen while the others run
+// on.  The iteration body is captured once in a HIP graph (no memset nodes)
+// and replayed; the host reads one counter per replay.
+// ---------------------------------------------------------------------------
+namespace psg {
+
+constexpr int kChunks = 32;  // workgroups per LP in the segmented kernels
+
+__global__ void __launch_bounds__(kBlock)
+k_bdot(const int* __restrict__ off, const double* __restrict__ a, const double* __restrict__ b,
+       double* __restrict__ part) {
+  const int lp = blockIdx.y;
+  const int r0 = off[lp], r1 = off[lp + 1];
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < r1; i += (int64_t)kChunks * blockDim.x)
+    acc[0] += a[i] * b[i];
+  block_sum<1>(acc);
+  if (threadIdx.x == 0) part[(int64_t)lp * kChunks + blockIdx.x] = acc[0];
+}
+
+__device__ inline double lp_sum(const double* __restrict__ part, int lp) {
+  double s = 0.0;
+  for (int k = 0; k < kChunks; ++k) s += part[(int64_t)lp * kChunks + k];
+  return s;
+}
+
+// scale = max(1, |rhs|), first check, rz0 = r.z with the guard of SSP:486-488
+__global__ void k_binit(int nlp, const double* __restrict__ part_bb, const double* __restrict__ part_rr,
+                        const double* __restrict__ part_rz, const double* __restrict__ tol, double* __restrict__ scale,
+                        double* __restrict__ rz, int* __restrict__ done, int* __restrict__ iters,
+                        int* __restrict__ info) {
+  for (int lp = blockIdx.x * blockDim.x + threadIdx.x; lp < nlp; lp += gridDim.x * blockDim.x) {
+    const double sc = fmax(1.0, sqrt(lp_sum(part_bb, lp)));
+    scale[lp] = sc;
+    iters[lp] = 0;
+    const double rel = sqrt(lp_sum(part_rr, lp)) / sc;
+    const double r0 = lp_sum(part_rz, lp);
+    rz[lp] = r0;
+    if (rel <= tol[lp]) { done[lp] = 1; info[lp] = 0; }
+    else if (!isfinite(r0) || r0 <= 0.0) { done[lp] = 1; info[lp] = -1; }
+    else { done[lp] = 0; info[lp] = 1; }
+  }
+}
+
+// alpha = rz / (p.q) with the curvature and step guards (info -2 / -3)
+__global__ void k_balpha(int nlp, const double* __restrict__ part_pq, const double* __restrict__ rz,
+                         double* __restrict__ alpha, int* __restrict__ done, int* __restrict__ info) {
+  for (int lp = blockIdx.x * blockDim.x + threadIdx.x; lp < nlp; lp += gridDim.x * blockDim.x) {
+    if (done[lp]) continue;
+    const double pq = lp_sum(part_pq, lp);
+    const double a = rz[lp] / pq;
+    if (!isfinite(pq) || pq <= 0.0) { info[lp] = -2; done[lp] = 1; continue; }
+    if (!isfinite(a) || a <= 0.0) { info[lp] = -3; done[lp] = 1; continue; }
+    alpha[lp] = a;
+  }
+}
+
+// x += alpha p, r -= alpha q on the LP's rows; partials of r.r
+__global__ void __launch_bounds__(kBlock)
+k_bstep(const int* __restrict__ off, const double* __restrict__ alpha, const int* __restrict__ done,
+        const double* __restrict__ p, const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r,
+        double* __restrict__ part) {
+  const int lp = blockIdx.y;
+  if (done[lp]) return;  // uniform per block
+  const int r0 = off[lp], r1 = off[lp + 1];
+  const double a = alpha[lp];
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < r1; i += (int64_t)kChunks * blockDim.x) {
+    x[i] += a * p[i];
+    const double ri = r[i] - a * q[i];
+    r[i] = ri;
+    acc[0] += ri * ri;
+  }
+  block_sum<1>(acc);
+  if (threadIdx.x == 0) part[(int64_t)lp * kChunks + blockIdx.x] = acc[0];
+}
+
+// count the step, freeze converged LPs and those at the iteration limit,
+// publish the number of frozen LPs (one block)
+__global__ void __launch_bounds__(kBlock)
+k_bcheck(int nlp, const double* __restrict__ part_rr, const double* __restrict__ scale, const double* __restrict__ tol,
+         int maxit, int* __restrict__ done, int* __restrict__ iters, int* __restrict__ info, int* __restrict__ ndone) {
+  double cnt[1] = {0.0};
+  for (int lp = threadIdx.x; lp < nlp; lp += blockDim.x) {
+    if (!done[lp]) {
+      const int it = ++iters[lp];
+      const double rel = sqrt(lp_sum(part_rr, lp)) / scale[lp];
+      if (rel <= tol[lp]) { done[lp] = 1; info[lp] = 0; }
+      else if (it >= maxit) { done[lp] = 1; info[lp] = maxit; }
+    }
+    cnt[0] += done[lp] ? 1.0 : 0.0;
+  }
+  block_sum<1>(cnt);
+  if (threadIdx.x == 0) *ndone = (int)cnt[0];
+}
+
+// number of frozen LPs (one block)
+__global__ void __launch_bounds__(kBlock) k_bcount(int nlp, const int* __restrict__ done, int* __restrict__ ndone) {
+  double cnt[1] = {0.0};
+  for (int lp = threadIdx.x; lp < nlp; lp += blockDim.x) cnt[0] += done[lp] ? 1.0 : 0.0;
+  block_sum<1>(cnt);
+  if (threadIdx.x == 0) *ndone = (int)cnt[0];
+}
+
+// beta = rz_new / rz with the guards of SSP:545-553 (info -6 / -7)
+__global__ void k_bbeta(int nlp, const double* __restrict__ part_rz, double* __restrict__ rz, double* __restrict__ beta,
+                        int* __restrict__ done, int* __restrict__ info) {
+  for (int lp = blockIdx.x * blockDim.x + threadIdx.x; lp < nlp; lp += gridDim.x * blockDim.x) {
+    if (done[lp]) continue;
+    const double rn = lp_sum(part_rz, lp);
+    const double b = rn / rz[lp];
+    if (!isfinite(rn) || rn <= 0.0) { info[lp] = -6; done[lp] = 1; continue; }
+    if (!isfinite(b) || b < 0.0) { info[lp] = -7; done[lp] = 1; continue; }
+    rz[lp] = rn;
+    beta[lp] = b;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_bdir(const int* __restrict__ off, const double* __restrict__ beta, const int* __restrict__ done,
+       const double* __restrict__ z, double* __restrict__ p) {
+  const int lp = blockIdx.y;
+  if (done[lp]) return;
+  const int r0 = off[lp], r1 = off[lp + 1];
+  const double b = beta[lp];
+  for (int64_t i = (int64_t)r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < r1; i += (int64_t)kChunks * blockDim.x)
+    p[i] = z[i] + b * p[i];
+}
+
+}  // namespace psg
+
+struct psg_batch {
+  psg_ctx* c = nullptr;
+  int nlp = 0;
+  psg::DevBuf<int> off, done, iters, info, ndone;
+  psg::DevBuf<double> rhs, x, r, z, p, q, rt, part, part2, part3, rz, alpha, beta, scale, tol;
+  int* hndone = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_iters = 0, graph_maxit = 0;
+  double last_ms = 0.0;
+  int launches = 0;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    graph_iters = 0;
+  }
+};
+
+namespace psg {
+
+// one PCG iteration of every LP (the captured body)
+static void batch_iteration(psg_batch* B, int maxit) {
+  psg_ctx* c = B->c;
+  hipStream_t s = c->stream;
+  const dim3 g2(kChunks, B->nlp);
+  const int gl = grid_for(B->nlp, kBlock, 64);
+  device_Hv(c, B->p.p, B->q.p, nullptr, true);
+  hipLaunchKernelGGL(k_bdot, g2, dim3(kBlock), 0, s, B->off.p, B->p.p, B->q.p, B->part.p);
+  hipLaunchKernelGGL(k_balpha, dim3(gl), dim3(kBlock), 0, s, B->nlp, B->part.p, B->rz.p, B->alpha.p, B->done.p, B->info.p);
+  hipLaunchKernelGGL(k_bstep, g2, dim3(kBlock), 0, s, B->off.p, B->alpha.p, B->done.p, B->p.p, B->q.p, B->x.p, B->r.p,
+                     B->part.p);
+  hipLaunchKernelGGL(k_bcheck, dim3(1), dim3(kBlock), 0, s, B->nlp, B->part.p, B->scale.p, B->tol.p, maxit, B->done.p,
+                     B->iters.p, B->info.p, B->ndone.p);
+  factor_apply(c, B->r.p, B->z.p);
+  hipLaunchKernelGGL(k_bdot, g2, dim3(kBlock), 0, s, B->off.p, B->r.p, B->z.p, B->part.p);
+  hipLaunchKernelGGL(k_bbeta, dim3(gl), dim3(kBlock), 0, s, B->nlp, B->part.p, B->rz.p, B->beta.p, B->done.p, B->info.p);
+  hipLaunchKernelGGL(k_bdir, g2, dim3(kBlock), 0, s, B->off.p, B->beta.p, B->done.p, B->z.p, B->p.p);
+  HIP_LAUNCH_CHECK();
+}
+
+}  // namespace psg
+
+extern "C" {
+
+int psg_batch_create(psg_ctx* c, int32_t nlp, const int64_t* row_off, psg_batch** out) {
+  return guarded([&]() -> int {
+    if (!c || nlp < 1 || !row_off || !out) throw Error(PSG_ERR_ARGUMENT, "psg_batch_create: invalid arguments");
+    if (nlp > 65535) throw Error(PSG_ERR_ARGUMENT, "psg_batch_create: at most 65535 LPs");
+    if (row_off[0] != 0 || row_off[nlp] != c->m) throw Error(PSG_ERR_ARGUMENT, "psg_batch_create: row offsets must span [0, m]");
+    std::vector<int> off(nlp + 1);
+    for (int k = 0; k <= nlp; ++k) {
+      if (k && row_off[k] < row_off[k - 1]) throw Error(PSG_ERR_ARGUMENT, "psg_batch_create: row offsets not monotone");
+      off[k] = (int)row_off[k];
+    }
+    auto* B = new psg_batch;
+    try {
+      B->c = c;
+      B->nlp = nlp;
+      B->off.upload(off, c->stream);
+      for (auto* v : {&B->rhs, &B->x, &B->r, &B->z, &B->p, &B->q, &B->rt}) v->alloc(c->m);
+      for (auto* v : {&B->part, &B->part2, &B->part3}) v->alloc((size_t)nlp * kChunks);
+      for (auto* v : {&B->rz, &B->alpha, &B->beta, &B->scale, &B->tol}) v->alloc(nlp);
+      for (auto* v : {&B->done, &B->iters, &B->info}) v->alloc(nlp);
+      B->ndone.alloc(1);
+      HIP_CHECK(hipHostMalloc((void**)&B->hndone, sizeof(int)));
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+    } catch (...) {
+      psg_batch_destroy(B);
+      throw;
+    }
+    *out = B;
+    return PSG_OK;
+  });
+}
+
+int psg_batch_destroy(psg_batch* B) {
+  if (!B) return PSG_OK;
+  if (B->c && B->c->stream) (void)hipStreamSynchronize(B->c->stream);
+  B->drop_graph();
+  if (B->hndone) (void)hipHostFree(B->hndone);
+  delete B;
+  return PSG_OK;
+}
+
+int psg_batch_pcg(psg_batch* B, const double* rhs, const double* tol, int32_t maxit, int32_t graph_iters,
+                  double* x_out, int32_t* iters_out, int32_t* info_out, double* relres_out) {
+  return guarded([&]() -> int {
+    if (!B || !rhs || !tol || maxit < 1 || graph_iters < 0 || !x_out || !iters_out || !info_out || !relres_out)
+      throw Error(PSG_ERR_ARGUMENT, "psg_batch_pcg: invalid arguments");
+    psg_ctx* c = B->c;
+    need_weights(c);
+    if (!factor_ready(c)) throw Error(PSG_ERR_STATE, "psg_batch_pcg: no factor");
+    hipStream_t s = c->stream;
+    const int nlp = B->nlp, m = c->m;
+    const dim3 g2(kChunks, nlp);
+    const int gl = grid_for(nlp, kBlock, 64);
+    HIP_CHECK(hipEventRecord(c->ev0, s));
+    B->rhs.upload(rhs, m, s);
+    B->tol.upload(tol, nlp, s);
+    // x0 = P^-1 rhs, r = rhs - H x0, z = P^-1 r, rz = r.z, p = z (SSP:471-488 per LP)
+    hipLaunchKernelGGL(k_bdot, g2, dim3(kBlock), 0, s, B->off.p, B->rhs.p, B->rhs.p, B->part.p);
+    factor_apply(c, B->rhs.p, B->x.p);
+    device_residual(c, B->rhs.p, B->x.p, nullptr, B->r.p, nullptr);
+    hipLaunchKernelGGL(k_bdot, g2, dim3(kBlock), 0, s, B->off.p, B->r.p, B->r.p, B->part2.p);
+    factor_apply(c, B->r.p, B->z.p);
+    hipLaunchKernelGGL(k_bdot, g2, dim3(kBlock), 0, s, B->off.p, B->r.p, B->z.p, B->part3.p);
+    hipLaunchKernelGGL(k_binit, dim3(gl), dim3(kBlock), 0, s, nlp, B->part.p, B->part2.p, B->part3.p, B->tol.p,
+                       B->scale.p, B->rz.p, B->done.p, B->iters.p, B->info.p);
+    HIP_CHECK(hipMemcpyAsync(B->p.p, B->z.p, (size_t)m * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_LAUNCH_CHECK();
+    int launches = 0, total = 0;
+    auto ndone = [&]() {
+      HIP_CHECK(hipMemcpyAsync(B->hndone, B->ndone.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      return *B->hndone;
+    };
+    hipLaunchKernelGGL(k_bcount, dim3(1), dim3(kBlock), 0, s, nlp, B->done.p, B->ndone.p);
+    HIP_LAUNCH_CHECK();
+    int nd = ndone();
+    if (graph_iters > 0) {
+      if (B->graph_iters != graph_iters || B->graph_maxit != maxit) {
+        B->drop_graph();
+        HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          for (int k = 0; k < graph_iters; ++k) batch_iteration(B, maxit);
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(s, &B->graph));
+        HIP_CHECK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+        B->graph_iters = graph_iters;
+        B->graph_maxit = maxit;
+      }
+      while (nd < nlp && total < maxit) {
+        HIP_CHECK(hipGraphLaunch(B->exec, s));
+        ++launches;
+        total += graph_iters;
+        nd = ndone();
+      }
+    } else {
+      while (nd < nlp && total < maxit) {
+        batch_iteration(B, maxit);
+        ++launches;
+        ++total;
+        nd = ndone();
+      }
+    }
+    // explicit final residuals per LP
+    device_residual(c, B->rhs.p, B->x.p, nullptr, B->rt.p, nullptr);
+    hipLaunchKernelGGL(k_bdot, g2, dim3(kBlock), 0, s, B->off.p, B->rt.p, B->rt.p, B->part.p);
+    HIP_LAUNCH_CHECK();
+    HIP_CHECK(hipEventRecord(c->ev1, s));
+    std::vector<double> part((size_t)nlp * kChunks), scale(nlp), tolv(tol, tol + nlp);
+    B->part.download(part.data(), part.size(), s);
+    B->scale.download(scale.data(), nlp, s);
+    B->x.download(x_out, m, s);
+    B->iters.download(iters_out, nlp, s);
+    B->info.download(info_out, nlp, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    B->last_ms = ms;
+    B->launches = launches;
+    if (factor_solve_error(c)) throw Error(PSG_ERR_INTERNAL, "triangular solve dependency wait timed out");
+    for (int lp = 0; lp < nlp; ++lp) {
+      double sm = 0.0;
+      for (int k = 0; k < kChunks; ++k) sm += part[(size_t)lp * kChunks + k];
+      relres_out[lp] = std::sqrt(sm) / scale[lp];
+      if (info_out[lp] > 0 && relres_out[lp] <= tolv[lp]) info_out[lp] = 0;
+    }
+    return PSG_OK;
+  });
+}
+
+int psg_batch_last(psg_batch* B, double* device_ms, int32_t* replays) {
+  return guarded([&]() -> int {
+    if (!B) throw Error(PSG_ERR_ARGUMENT, "psg_batch_last: null batch");
+    if (device_ms) *device_ms = B->last_ms;
+    if (replays) *replays = B->launches;
+    return PSG_OK;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Interior-point vector kernels: the products A v and A' y of the residuals
+// (PS:1984-1993, 2561-2562) and the fused step-to-boundary ratio tests
+// (PS:1831-1837 as used by primal_step / dual_step, PS:2101-2106) on the
+// iterate the driver hands to observe() once per iteration.
+// ---------------------------------------------------------------------------
+namespace psg {
+
+__device__ inline double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ inline void block_min_write(double v, double* __restrict__ out) {
+  __shared__ double sh[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  v = wave_min(v);
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = sh[0];
+    for (int k = 1; k < kBlock / kWave; ++k) r = fmin(r, sh[k]);
+    *out = r;
+  }
+}
+
+// primal: min over pos with dx < 0 of -x/dx and over box with -dx < 0 of -s/(-dx)
+__global__ void __launch_bounds__(kBlock)
+k_ipm_step_primal(int n, const double* __restrict__ x, const double* __restrict__ s, const double* __restrict__ dx,
+                  const uint8_t* __restrict__ pos, const uint8_t* __restrict__ box, double* __restrict__ part) {
+  double r = INFINITY;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double d = dx[i];
+    if (pos[i] && d < 0.0) r = fmin(r, -x[i] / d);
+    if (box[i] && -d < 0.0) r = fmin(r, -s[i] / (-d));
+  }
+  block_min_write(r, part + blockIdx.x);
+}
+
+// dual: min over pos with dz < 0 of -z/dz and over box with dw < 0 of -w/dw
+__global__ void __launch_bounds__(kBlock)
+k_ipm_step_dual(int n, const double* __restrict__ z, const double* __restrict__ w, const double* __restrict__ dz,
+                const double* __restrict__ dw, const uint8_t* __restrict__ pos, const uint8_t* __restrict__ box,
+                double* __restrict__ part) {
+  double r = INFINITY;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double a = dz[i];
+    if (pos[i] && a < 0.0) r = fmin(r, -z[i] / a);
+    const double b = dw ? dw[i] : 0.0;
+    if (box[i] && b < 0.0) r = fmin(r, -w[i] / b);
+  }
+  block_min_write(r, part + blockIdx.x);
+}
+
+static double read_min(psg_ctx* c, const double* part, int nb) {
+  std::vector<double> h(nb);
+  HIP_CHECK(hipMemcpyAsync(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  double r = INFINITY;
+  for (double v : h) r = std::fmin(r, v);
+  return r;
+}
+
+}  // namespace psg
+
+extern "C" {
+
+int psg_spmv(psg_ctx* c, int32_t trans, const double* in, double* out) {
+  return guarded([&]() -> int {
+    if (!c || !in || !out || (trans != 0 && trans != 1)) throw Error(PSG_ERR_ARGUMENT, "psg_spmv: invalid arguments");
+    hipStream_t s = c->stream;
+    if (trans) {  // out (n) = A' in (m): column gather, no scaling
+      c->vin.upload(in, c->m, s);
+      launch_gather<double>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval.p, c->vin.p, nullptr, nullptr, c->dx_n.p, s);
+      c->dx_n.download(out, c->n, s);
+    } else {  // out (m) = A in (n): row gather over the column blocks
+      c->t_n.upload(in, c->n, s);
+      launch_csr<double, kModeHv>(c, c->rval.p, c->t_n.p, c->vin.p, 0.0, nullptr, nullptr, c->vout.p, nullptr, s);
+      c->vout.download(out, c->m, s);
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    return PSG_OK;
+  });
+}
+
+int psg_ipm_observe(psg_ctx* c, const double* x, const double* z, const double* s_, const double* w,
+                    const uint8_t* pos, const uint8_t* box) {
+  return guarded([&]() -> int {
+    if (!c || !x || !z || !s_ || !w) throw Error(PSG_ERR_ARGUMENT, "psg_ipm_observe: null argument");
+    hipStream_t s = c->stream;
+    c->ipm_x.upload(x, c->n, s);
+    c->ipm_z.upload(z, c->n, s);
+    c->ipm_s.upload(s_, c->n, s);
+    c->ipm_w.upload(w, c->n, s);
+    if (pos) c->ipm_pos.upload(pos, c->n, s);
+    if (box) c->ipm_box.upload(box, c->n, s);
+    if (!c->ipm_pos.p || !c->ipm_box.p) throw Error(PSG_ERR_ARGUMENT, "psg_ipm_observe: masks never supplied");
+    c->ipm_d1.grow(c->n);
+    c->ipm_d2.grow(c->n);
+    HIP_CHECK(hipStreamSynchronize(s));
+    c->have_iterate = true;
+    return PSG_OK;
+  });
+}
+
+int psg_ipm_step(psg_ctx* c, int32_t dual, const double* d1, const double* d2, double* ratio) {
+  return guarded([&]() -> int {
+    if (!c || !d1 || !ratio || (dual != 0 && dual != 1)) throw Error(PSG_ERR_ARGUMENT, "psg_ipm_step: invalid arguments");
+    if (!c->have_iterate) throw Error(PSG_ERR_STATE, "psg_ipm_step: call psg_ipm_observe first");
+    hipStream_t s = c->stream;
+    const int grid = grid_for(c->n, kBlock, kRedBlocks);
+    c->ipm_d1.upload(d1, c->n, s);
+    if (dual) {
+      if (d2) c->ipm_d2.upload(d2, c->n, s);
+      hipLaunchKernelGGL(k_ipm_step_dual, dim3(grid), dim3(kBlock), 0, s, c->n, c->ipm_z.p, c->ipm_w.p, c->ipm_d1.p,
+                         d2 ? c->ipm_d2.p : nullptr, c->ipm_pos.p, c->ipm_box.p, c->partD.p);
+    } else {
+      hipLaunchKernelGGL(k_ipm_step_primal, dim3(grid), dim3(kBlock), 0, s, c->n, c->ipm_x.p, c->ipm_s.p, c->ipm_d1.p,
+                         c->ipm_pos.p, c->ipm_box.p, c->partD.p);
+    }
+    HIP_LAUNCH_CHECK();
+    *ratio = std::fmin(1.0, read_min(c, c->partD.p, grid));
     return PSG_OK;
   });
 }

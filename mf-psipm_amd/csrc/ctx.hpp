@@ -25,7 +25,6 @@ struct Block2 {
   DevBuf<double> rhs, x, r, z, p, q, rt;  // 2 m
   DevBuf<double> tn, r1n, dxn;            // 2 n
   DevBuf<double> partA, partC, partD, hvdots, scal, rzs, alphas, betas;
-  DevBuf<int> mode;                       // per-column direction mode
   double* hscal = nullptr;                // pinned mirror of scal
   bool have_rhs = false, have_solution = false;
   ~Block2() {
@@ -77,6 +76,10 @@ struct psg_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   psg::FactorState* fs = nullptr;
   psg::Block2* b2 = nullptr;
+  // interior-point iterate for the step-length kernels (psg_ipm_observe)
+  psg::DevBuf<double> ipm_x, ipm_z, ipm_s, ipm_w, ipm_d1, ipm_d2;
+  psg::DevBuf<uint8_t> ipm_pos, ipm_box;
+  bool have_iterate = false;
 };
 
 namespace psg {

@@ -1290,6 +1290,22 @@ k_tail_upper(int r, const int* __restrict__ tiles, const T* __restrict__ X, cons
   }
 }
 
+// Clearing kernels: the solve path clears its counters and accumulators
+// with kernels rather than memset nodes, so the whole apply can be captured
+// in a HIP graph and replayed.
+template <class T>
+__global__ void k_zero(T* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = T(0);
+}
+
+template <class T>
+static void zero_async(T* p, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_zero<T>), dim3(grid_for(n, kBlock, 2048)), dim3(kBlock), 0, st, p, n);
+  HIP_LAUNCH_CHECK();
+}
+
 __global__ void k_fill_int(int* __restrict__ p, int n, int v) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
@@ -1476,11 +1492,11 @@ static void tail_solve(psg_ctx* c, const T* X, T* vb, T* vy, T* vx) {
   FactorState* fs = c->fs;
   hipStream_t st = c->stream;
   const int r = fs->tail_r, F = fs->tail_F;
-  HIP_CHECK(hipMemsetAsync(vy + (size_t)F * NR, 0, (size_t)r * NR * sizeof(T), st));
+  zero_async(vy + (size_t)F * NR, (int64_t)r * NR, st);
   hipLaunchKernelGGL((k_tail_lower<T, NR>), dim3(fs->tail_nlower), dim3(kBlock), 0, st, r, fs->tail_lower_tiles.p, X,
                      vb + (size_t)F * NR, vy + (size_t)F * NR);
   HIP_LAUNCH_CHECK();
-  HIP_CHECK(hipMemsetAsync(vx + (size_t)F * NR, 0, (size_t)r * NR * sizeof(T), st));
+  zero_async(vx + (size_t)F * NR, (int64_t)r * NR, st);
   hipLaunchKernelGGL((k_tail_upper<T, NR>), dim3(fs->tail_nupper), dim3(kBlock), 0, st, r, fs->tail_upper_tiles.p, X,
                      vy + (size_t)F * NR, vx + (size_t)F * NR);
   HIP_LAUNCH_CHECK();
@@ -1905,7 +1921,7 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   hipStream_t s = c->stream;
   const int m = c->m;
   const int ns = fs->nsuper;
-  HIP_CHECK(hipMemsetAsync(fs->flags.p, 0, fs->flags.n * sizeof(int), s));
+  zero_async(fs->flags.p, (int64_t)fs->flags.n, s);
   hipLaunchKernelGGL((k_perm_in<T, NR>), dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, fs->perm.p, c->dsc.p,
                      r, vb);
   HIP_LAUNCH_CHECK();
@@ -1939,7 +1955,7 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   a.sleep_max = fs->sf_sleep;
   a.batch = fs->sf_batch;
   a.trace = fs->trace.p;
-  HIP_CHECK(hipMemsetAsync(a.acc, 0, (size_t)m * NR * sizeof(T), s));
+  zero_async(a.acc, (int64_t)m * NR, s);
   const int grid = std::max(1, std::min(fs->sf_units, fs->sf_grid));
   if (fs->n_iso) {
     hipLaunchKernelGGL((k_iso_solve<T, NR>), dim3(grid_for(fs->n_iso, kBlock, 4096)), dim3(kBlock), 0, s, fs->n_iso,

@@ -143,10 +143,15 @@ def prox_eval(
     rmz = np.zeros(n)
     rmw = np.zeros(n)
 
+    Ax = getattr(kkt, "matvec", None) or (lambda v: A @ v)
+    Aty = getattr(kkt, "rmatvec", None) or (lambda v: At @ v)
+    gpu_step = getattr(kkt, "ipm_step", None)
+
     def residuals():
-        rd = -(c - At @ y - z + w + rho * (x - xk))
-        prox_p = A @ x - b + delta * (y - yk)
-        g = c - At @ y + rho * (x - xk)
+        aty = Aty(y)
+        rd = -(c - aty - z + w + rho * (x - xk))
+        prox_p = Ax(x) - b + delta * (y - yk)
+        g = c - aty + rho * (x - xk)
         gp = g.copy()
         gp[ipos] = x[ipos] - np.minimum(np.maximum(x[ipos] - g[ipos], 0.0), u[ipos])
         return rd, -prox_p, np.linalg.norm(gp) + np.linalg.norm(prox_p)
@@ -197,9 +202,13 @@ def prox_eval(
         return out
 
     def primal_step(d):
+        if gpu_step is not None:
+            return tau * gpu_step(0, d)
         return tau * min(step_to_boundary(x, d, ipos), step_to_boundary(s, -d, ibox))
 
     def dual_step(dz_, dw_):
+        if gpu_step is not None:
+            return tau * gpu_step(1, dz_, dw_ if nbox else None)
         return tau * min(step_to_boundary(z, dz_, ipos), step_to_boundary(w, dw_, ibox))
 
     while iters < maxit:
@@ -604,9 +613,13 @@ def psipm_solve(
     sol = Solution(x=x, y=y, z=z, w=w)
     inner_total = 0
     it = 0
+    Ax = getattr(solver, "matvec", None) or (lambda v: A @ v)
+    Aty = getattr(solver, "rmatvec", None) or (lambda v: At @ v)
+    if hasattr(solver, "ipm_setup"):
+        solver.ipm_setup(lp.pos, lp.box)
     for it in range(1, maxit + 1):
-        rp = b - A @ x
-        rd = c - At @ y - z + w
+        rp = b - Ax(x)
+        rd = c - Aty(y) - z + w
         s = np.ones(n)
         if nbox:
             sync_box(x, s, u, ibox, x_floor, s_floor)

@@ -251,6 +251,8 @@ class HipSpectralStableColumnPreconditioner:
         self._iteration = 0
         self.learned_refreshes = self.learned_fallbacks = 0
         self.pair_solves = self.pair_fallbacks = 0
+        self._ipm_masks = None
+        self._ipm_masks_sent = False
         # device-time accounting (HIP events)
         self.device_pcg_ms = 0.0
         self.device_factor_ms = 0.0
@@ -309,8 +311,42 @@ class HipSpectralStableColumnPreconditioner:
         features."""
         self._obs = (x, z, float(mu))
         self._iteration += 1
+        if self._ipm_masks is not None:
+            pos, box = self._ipm_masks if not self._ipm_masks_sent else (None, None)
+            self._call("psg_ipm_observe", self._ctx, np.ascontiguousarray(x, dtype=float),
+                       np.ascontiguousarray(z, dtype=float), np.ascontiguousarray(s, dtype=float),
+                       np.ascontiguousarray(w, dtype=float), pos, box)
+            self._ipm_masks_sent = True
         nm = max(float(mu), 0.0) / (1.0 + max(float(mu), 0.0))
         self.current_cg_tol = min(self.cg_tol_max, max(self.cg_tol, min(self.forcing_factor * nm, self._ew_tolerance)))
+
+    # ------------------------------------------------- interior-point kernels
+    def ipm_setup(self, pos, box):
+        """Masks of the nonnegative and boxed columns for ``ipm_step``; from
+        then on ``observe`` keeps the iterate on the device."""
+        self._ipm_masks = (np.ascontiguousarray(pos, dtype=np.uint8), np.ascontiguousarray(box, dtype=np.uint8))
+        self._ipm_masks_sent = False
+
+    def ipm_step(self, dual, d1, d2=None):
+        """Step to the boundary on the GPU for the iterate of the last observe():
+        dual 0 -> min(stb(x, d1, pos), stb(s, -d1, box)); dual 1 ->
+        min(stb(z, d1, pos), stb(w, d2, box))  (PS:1831-1837)."""
+        out = C.c_double()
+        self._call("psg_ipm_step", self._ctx, int(dual), np.ascontiguousarray(d1, dtype=float),
+                   None if d2 is None else np.ascontiguousarray(d2, dtype=float), C.byref(out))
+        return float(out.value)
+
+    def matvec(self, v):
+        """A v on the GPU."""
+        out = np.empty(self.m)
+        self._call("psg_spmv", self._ctx, 0, np.ascontiguousarray(v, dtype=float), out)
+        return out
+
+    def rmatvec(self, y):
+        """A' y on the GPU."""
+        out = np.empty(self.n)
+        self._call("psg_spmv", self._ctx, 1, np.ascontiguousarray(y, dtype=float), out)
+        return out
 
     def _linear_tolerance(self):
         tol = self.current_cg_tol
@@ -767,6 +803,7 @@ class HipSpectralStableColumnPreconditioner:
         self._call("psg_solve_begin2", self._ctx, r1, r2)
         tols = [self._linear_tolerance(), self._linear_tolerance()]
         results = self._pcg_pair(tols)
+        self.last_pair_results = results
         ok = []
         for q, res in enumerate(results):
             tol = tols[q]

@@ -1,0 +1,57 @@
+"""Train the learned column selector on trajectories the solver generates.
+
+Usage (on a GPU host):  python mf-psipm_amd/train_selector.py [--m 2000 --n 100000
+                         --seeds 1000 1001 1002 --steps 1500 --out PATH]
+
+Runs the rule-based GPU solver on set-covering LPs of the training seeds
+(disjoint from the evaluation seeds 0..4), records the selector features of
+every factorisation and labels them with the rule's pick at the next
+iteration, trains the MLP with the library's kernels and saves
+``params``/``norm`` (float32 .npz).
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import numpy as np  # noqa: E402
+
+import mf_psipm_amd  # noqa: E402,F401
+from mf_psipm_amd.selector import LearnedColumnSelector, collect_trajectories, setcover_problem  # noqa: E402
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--m", type=int, default=2000)
+    ap.add_argument("--n", type=int, default=100000)
+    ap.add_argument("--seeds", type=int, nargs="+", default=[1000, 1001, 1002])
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", default=os.path.join(HERE, "models", "selector_setcover.npz"))
+    args = ap.parse_args()
+    if min(args.seeds) < 1000:
+        raise SystemExit("training seeds must be >= 1000 (0..999 are evaluation seeds)")
+    t0 = time.perf_counter()
+    problems = [setcover_problem(args.m, args.n, s) for s in args.seeds]
+    X, y = collect_trajectories([(lp, tol) for _, lp, tol in problems], structural_backbone=False)
+    print(f"trajectories: {X.shape[0]} samples, positive fraction {y.mean():.3f}, "
+          f"{time.perf_counter() - t0:.1f}s", flush=True)
+    sel = LearnedColumnSelector(seed=0)
+    t0 = time.perf_counter()
+    losses = sel.fit(X, y, steps=args.steps, batch=args.batch, lr=args.lr)
+    acc = float(np.mean((sel.forward(X) > 0) == (y > 0.5)))
+    print(f"training: {args.steps} Adam steps in {time.perf_counter() - t0:.2f}s, loss "
+          f"{losses[:20].mean():.4f} -> {losses[-50:].mean():.4f}, accuracy {acc:.4f}", flush=True)
+    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    sel.save(args.out)
+    print(f"saved {args.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -86,9 +86,11 @@ def test_solve_pair_matches_single_solves_and_oracle(gpu, precision):
     _say(f"solve_pair {precision}")
     m, n = 1600, 4800
     _, lp = _lp(m, n, 5, 31)
+    # enough iterations that neither column needs the retry ladder
     g = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, precision=precision,
-                                              cg_tol=1e-9, cg_tol_max=1e-9)
-    c = SpectralStableColumnPreconditionerCPU(lp.A, protected=~lp.pos, cg_tol=1e-9, cg_tol_max=1e-9)
+                                              cg_tol=1e-9, cg_tol_max=1e-9, cg_maxit=2000)
+    c = SpectralStableColumnPreconditionerCPU(lp.A, protected=~lp.pos, cg_tol=1e-9, cg_tol_max=1e-9,
+                                              cg_maxit=2000)
     theta = _theta(n, 33)
     g.factorize(theta, 1e-6)
     c.factorize(theta, 1e-6)
@@ -97,6 +99,9 @@ def test_solve_pair_matches_single_solves_and_oracle(gpu, precision):
     r1b, r2b = np.zeros(n), 100.0 * rng.standard_normal(m)  # different scales and convergence
     (dxa, dya), (dxb, dyb) = g.solve_pair(r1a, r2a, r1b, r2b)
     its_pair = list(g.cg_iterations[-2:])
+    for q, r in enumerate(g.last_pair_results):
+        _say(f"column {q}: info {r.info} iterations {r.iterations} relres {r.relative_residual:.3e} "
+             f"replacements {r.replacements} checks {r.residual_checks}")
     assert g.pair_solves == 1 and g.pair_fallbacks == 0
     H = lambda v: lp.A @ ((1 / theta) * (lp.A.T @ v)) + 1e-6 * v
     for r1, r2, dx, dy in ((r1a, r2a, dxa, dya), (r1b, r2b, dxb, dyb)):
