@@ -160,6 +160,9 @@ int psg_apply_P2(psg_ctx* ctx, const double* r, double* z);
  * launch of each hot-path kernel per step for all LPs; an LP is frozen once
  * its recurrence residual meets tol[lp].  With graph_iters > 0 the step is
  * captured once as a HIP graph of graph_iters iterations and replayed.
+ * The operator H is applied with FP64 values in both precisions, so a
+ * frozen LP's recurrence residual is its FP64 residual; PSG_MIXED keeps the
+ * FP32 factor and triangular solves.
  * Outputs: x (m), per-LP iterations, info, explicit relative residual. */
 typedef struct psg_batch psg_batch;
 int psg_batch_create(psg_ctx* ctx, int32_t nlp, const int64_t* row_off, psg_batch** out);

@@ -1509,7 +1509,10 @@ static void batch_iteration(psg_batch* B, int maxit) {
   hipStream_t s = c->stream;
   const dim3 g2(kChunks, B->nlp);
   const int gl = grid_for(B->nlp, kBlock, 64);
-  device_Hv(c, B->p.p, B->q.p, nullptr, true);
+  // FP64 operator values in every precision: a frozen LP's recurrence
+  // residual is then its true FP64 residual up to rounding (the mixed mode
+  // keeps its FP32 factor and triangular solves)
+  device_Hv(c, B->p.p, B->q.p, nullptr, false);
   hipLaunchKernelGGL(k_bdot, g2, dim3(kBlock), 0, s, B->off.p, B->p.p, B->q.p, B->part.p);
   hipLaunchKernelGGL(k_balpha, dim3(gl), dim3(kBlock), 0, s, B->nlp, B->part.p, B->rz.p, B->alpha.p, B->done.p, B->info.p);
   hipLaunchKernelGGL(k_bstep, g2, dim3(kBlock), 0, s, B->off.p, B->alpha.p, B->done.p, B->p.p, B->q.p, B->x.p, B->r.p,

@@ -115,7 +115,7 @@ class HipSpectralStableColumnPreconditioner:
         minres_max_dimension=200_000, fail_tol=1e-6, low_rank_max_columns=8,
         low_rank_min_nnz=1024, low_rank_clique_ratio=2.0, ordering=1, max_width=64,
         dense_factor=10.0, max_factor_entries: Optional[float] = None, error_class=KKTSolveError,
-        selector=None, selector_patience=2, recorder=None,
+        selector=None, selector_patience=2, selector_floor=0.05, recorder=None,
         **_ignored,
     ):
         if core_factor <= 0.0:
@@ -237,12 +237,14 @@ class HipSpectralStableColumnPreconditioner:
         self.residual_replacements = self.residual_checks = self.stagnation_events = 0
         self.core_swaps = self.minres_fallbacks = self.minres_iterations = 0
         self.krylov_breakdowns = self.krylov_iteration_limits = 0
-        # learned column selection (no counterpart in the reference): the
-        # network's scores replace the leverage scores at core refreshes until
+        # learned column selection (no counterpart in the reference): at core
+        # refreshes the leverage scores are multiplied by (floor + p_j), p_j the
+        # network's probability that column j ends basic, until
         # `selector_patience` learned refreshes in a row are followed by a
-        # poor solve; then the rule takes over (fallback)
+        # poor solve; then the plain rule takes over (fallback)
         self.selector = selector
         self.selector_patience = max(1, int(selector_patience))
+        self.selector_floor = max(0.0, float(selector_floor))
         self.recorder = recorder
         self._learned_active = selector is not None
         self._learned_pending = False
@@ -425,14 +427,12 @@ class HipSpectralStableColumnPreconditioner:
         return 1.0 / (1.0 + np.exp(-logits.astype(float)))
 
     def _record_trajectory(self, weights, delta):
-        """Features now, labelled later with the rule's pick (TrajectoryRecorder)."""
+        """Features of this factorisation for the TrajectoryRecorder (labelled
+        when the run ends)."""
         if self._obs is None or not self.candidates.size:
             return
-        scores = self._scores(weights, delta)
-        target = min(self.core_target, self.candidates.size)
-        pick = self.candidates[top_k(scores, target)]
         x, z, mu = self._obs
-        self.recorder.record(self.candidates, np.sort(pick), self.selector_features(x, z, mu, self._iteration))
+        self.recorder.record(self.candidates, self.selector_features(x, z, mu, self._iteration))
 
     def _refresh_core(self, weights, delta, stable):
         """Mandatory + retained + top-score fill (SAP:311-371, SSP:388-394)."""
@@ -442,12 +442,14 @@ class HipSpectralStableColumnPreconditioner:
             self.core = self.candidates.copy()
             self.retained_fractions.append(1.0)
         else:
-            scores = self._learned_scores() if self.selector is not None else None
-            self._learned_pending = scores is not None
-            if scores is not None:
+            scores = self._scores(weights, delta)
+            prob = self._learned_scores() if self.selector is not None else None
+            self._learned_pending = prob is not None
+            if prob is not None:
+                # the network reweights the rule: columns it expects to end
+                # basic keep their leverage, the others are demoted
+                scores = scores * (self.selector_floor + prob)
                 self.learned_refreshes += 1
-            else:
-                scores = self._scores(weights, delta)
             target = min(self.core_target, self.candidates.size)
             picked = []
             mand_local = np.searchsorted(self.candidates, self.mandatory)

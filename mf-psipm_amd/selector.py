@@ -3,8 +3,8 @@
 The reference picks the preconditioner's core columns by a leverage rule
 (psipm_stable_preconditioner.py:293-371).  ``LearnedColumnSelector`` adds a
 per-column MLP (8 -> 64 -> 64 -> 1, ReLU) that predicts which candidate
-columns the rule will put in the core at the next interior-point iteration,
-from each column's trajectory: log10 theta_j at the last three
+columns end basic (the active set the core should hold as the method
+converges), from each column's trajectory: log10 theta_j at the last three
 factorisations, log10 x_j, log10 z_j, log10 ||a_j||_2, log10 mu and the
 iteration index.  Inference (BF16 MFMA, FP32 accumulation), training (FP32
 forward/backward, Adam) and the feature construction are HIP kernels of
@@ -12,8 +12,8 @@ libpsipm_hip.so; this module holds the parameters, the trajectory recorder
 and the training driver.
 
 ``TrajectoryRecorder`` is attached to the GPU preconditioner object; on every
-factorisation it records the features of a sample of candidate columns and,
-one iteration later, labels them with the rule's core of that iteration.
+factorisation it records the features of a sample of candidate columns, and
+labels them with the final basic set when the run ends.
 """
 from __future__ import annotations
 
@@ -124,35 +124,41 @@ class TrajectoryRecorder:
     """Collects (features, label) pairs along an interior-point run.
 
     At each factorisation the preconditioner object calls ``record`` with the
-    rule's core of that iteration and the features of all candidates.  The
-    sample recorded at iteration k is labelled with the core of iteration
-    k + 1 ("next-iteration active columns").  Per iteration at most
-    ``per_iter`` columns are kept: every column of either core plus a
-    uniform sample of the rest.
+    candidate columns and their features.  When the run ends, ``end_run``
+    labels every recorded sample with whether its column is basic in the
+    final iterate (x_j > z_j) -- the columns the preconditioner's core should
+    hold as the method converges.  Per iteration at most ``per_iter`` columns
+    are kept: a uniform sample, with the columns of the largest x_j / z_j
+    (likely basic) always included.
     """
 
     def __init__(self, per_iter: int = 20000, seed: int = 0):
         self.per_iter = int(per_iter)
         self.rng = np.random.default_rng(seed)
-        self._pending = None
+        self._run = []
         self.X, self.y = [], []
 
-    def record(self, candidates: np.ndarray, core: np.ndarray, features: np.ndarray):
-        in_core = np.isin(candidates, core, assume_unique=True)
-        if self._pending is not None:
-            rows, feats = self._pending
-            self.X.append(feats)
-            self.y.append(in_core[rows].astype(np.float32))
-        # rows to keep for the next label: current core members and a sample
-        core_rows = np.flatnonzero(in_core)
-        rest = np.flatnonzero(~in_core)
-        k = max(0, min(rest.size, self.per_iter - core_rows.size))
-        pick = np.concatenate([core_rows, self.rng.choice(rest, size=k, replace=False) if k else rest[:0]])
-        self._pending = (pick, features[pick].copy())
+    def record(self, candidates: np.ndarray, features: np.ndarray):
+        k = candidates.size
+        if k <= self.per_iter:
+            rows = np.arange(k)
+        else:
+            # features 3, 4 are log10 x_j, log10 z_j: keep the top half by x/z
+            ratio = features[:, 3] - features[:, 4]
+            top = np.argpartition(-ratio, self.per_iter // 2)[: self.per_iter // 2]
+            rest = self.rng.choice(k, size=self.per_iter - top.size, replace=False)
+            rows = np.unique(np.concatenate([top, rest]))
+        self._run.append((candidates[rows].copy(), features[rows].copy()))
 
-    def end_run(self):
-        """Drop the unlabelled sample of the last iteration of a run."""
-        self._pending = None
+    def end_run(self, x=None, z=None):
+        """Label the run's samples with the final basic set (x_j > z_j); without
+        an iterate the run is dropped."""
+        if x is not None and z is not None:
+            basic = np.asarray(x) > np.asarray(z)
+            for cols, feats in self._run:
+                self.X.append(feats)
+                self.y.append(basic[cols].astype(np.float32))
+        self._run = []
 
     def dataset(self):
         if not self.X:
@@ -201,6 +207,6 @@ def collect_trajectories(problems, per_iter=20000, seed=0, **opts):
     """Run the rule-based solver on (lp, tol) pairs and record (features, labels)."""
     rec = TrajectoryRecorder(per_iter=per_iter, seed=seed)
     for lp, tol in problems:
-        solve_with(lp, tol, recorder=rec, **opts)
-        rec.end_run()
+        sol, _ = solve_with(lp, tol, recorder=rec, **opts)
+        rec.end_run(sol.x, sol.z) if sol.status == "optimal" else rec.end_run()
     return rec.dataset()

@@ -5,8 +5,8 @@ Usage (on a GPU host):  python mf-psipm_amd/train_selector.py [--m 2000 --n 1000
 
 Runs the rule-based GPU solver on set-covering LPs of the training seeds
 (disjoint from the evaluation seeds 0..4), records the selector features of
-every factorisation and labels them with the rule's pick at the next
-iteration, trains the MLP with the library's kernels and saves
+every factorisation and labels them with the run's final basic set
+(x_j > z_j), trains the MLP with the library's kernels and saves
 ``params``/``norm`` (float32 .npz).
 """
 from __future__ import annotations

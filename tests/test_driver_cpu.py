@@ -1,0 +1,88 @@
+"""CPU tests of the host additions around the hot path: the pc = 4 driver
+mode (predictor and centring direction solved as a pair), the trajectory
+recorder of the learned selector, and the selector's C-ABI error path.
+
+pc = 4 runs here with the CPU oracle, which has no ``solve_pair`` and takes
+the driver's two-solve branch; the optimum is pinned by scipy's HiGHS.
+"""
+import numpy as np
+import pytest
+from scipy.optimize import linprog
+
+import mf_psipm_amd as P
+from mf_psipm_amd import _lib
+from mf_psipm_amd.selector import TrajectoryRecorder
+from oracle import ssp_cpu as O
+
+
+def _lp(m, n, k, seed):
+    raw, *_ = P.generators.random_lp(m, n, k, seed)
+    lp = P.lp.to_standard_form(raw)
+    P.lp.scale_problem(lp)
+    return raw, lp
+
+
+class _PairCounting(O.SpectralStableColumnPreconditionerCPU):
+    """Oracle object with a solve_pair made of two solves (counts the calls)."""
+
+    pairs = 0
+
+    def solve_pair(self, r1a, r2a, r1b, r2b):
+        self.pairs += 1
+        return self.solve(r1a, r2a), self.solve(r1b, r2b)
+
+
+@pytest.mark.parametrize("with_pair", [False, True])
+def test_pc4_driver_matches_highs(with_pair):
+    raw, lp = _lp(300, 900, 5, 3)
+    tol = P.lp.scaled_tolerance(lp, 1e-8)
+    cls = _PairCounting if with_pair else O.SpectralStableColumnPreconditionerCPU
+    kkt = cls(lp.A, protected=~lp.pos, cg_tol=1e-8, cg_tol_max=1e-8, full_preconditioner_row_threshold=0)
+    sol = P.ipm.psipm_solve(lp, kkt, tol=tol, compl="gap", maxit=200, pc=4)
+    info = P.lp.recover(lp, raw, sol.x, sol.y, sol.z, sol.w)
+    ref = linprog(raw.c, A_eq=raw.A, b_eq=raw.rl, bounds=(0, None), method="highs-ds")
+    assert sol.status == "optimal"
+    assert abs(info["obj"] - ref.fun) <= 1e-8 * abs(ref.fun)
+    assert max(info["pinf"], info["dinf"]) <= 1e-8
+    if with_pair:
+        assert kkt.pairs > 0
+
+
+def test_trajectory_recorder_labels_final_basic_set():
+    rng = np.random.default_rng(0)
+    cand = np.arange(0, 200, 2)  # candidate column indices
+    rec = TrajectoryRecorder(per_iter=40, seed=1)
+    for it in range(3):
+        feats = rng.standard_normal((cand.size, 8)).astype(np.float32)
+        feats[:, 0] = cand  # tag each row with its column
+        rec.record(cand, feats)
+    x = rng.uniform(0, 1, 200)
+    z = rng.uniform(0, 1, 200)
+    rec.end_run(x, z)
+    X, y = rec.dataset()
+    assert X.shape == (y.size, 8) and 3 * 20 <= y.size <= 3 * 40
+    cols = X[:, 0].astype(int)
+    assert np.array_equal(y, (x > z)[cols].astype(np.float32))
+    # a run that ends without a final iterate is dropped
+    rec.record(cand, np.zeros((cand.size, 8), np.float32))
+    rec.end_run()
+    assert rec.dataset()[0].shape[0] == X.shape[0]
+
+
+def test_trajectory_recorder_keeps_likely_basic_columns():
+    cand = np.arange(1000)
+    feats = np.zeros((1000, 8), np.float32)
+    feats[:, 3] = np.linspace(-5, 5, 1000)  # log10 x rising with the index
+    rec = TrajectoryRecorder(per_iter=100, seed=0)
+    rec.record(cand, feats)
+    cols, _ = rec._run[0]
+    assert set(range(950, 1000)) <= set(cols.tolist())
+
+
+def test_selector_create_fails_without_gpu(lib):
+    import ctypes as C
+    if _lib.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    h = C.c_void_p()
+    assert lib.psg_mlp_create(C.byref(h)) == -2
+    assert b"no GPU" in lib.psg_last_error()
