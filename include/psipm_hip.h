@@ -114,8 +114,17 @@ int psg_scores(psg_ctx* ctx, const int32_t* candidates, int64_t ncand, double de
  * The plan is uploaded on first use and cached while it stays the same. */
 int psg_factor(psg_ctx* ctx, const psg_plan* plan, double eta, double* nnz_factor);
 
-/* z = D P^T L^-T L^-1 P D r  -- _apply_base (PS:903-904) */
+/* z = D P^T L^-T L^-1 P D r  -- _apply_base (PS:903-904), followed by the
+ * Woodbury correction when one is set (_apply, PS:949-956) */
 int psg_apply_P(psg_ctx* ctx, const double* r, double* z);
+/* Low-rank (Woodbury) columns, PS:691-717 and 906-956.  q <= 8 columns
+ * `cols` of A, U = A[:, cols] dense (m x q, column-major).  They leave the
+ * factor's dropped diagonal.  With B = M0^-1 U (m x q, from q solves of the
+ * base preconditioner) and Cinv = (Theta_L + U' B)^-1 (q x q, column-major)
+ * every preconditioner solve is corrected by z -= B Cinv U' z; with B or
+ * Cinv NULL the correction is off (for computing B).  q = 0 clears. */
+int psg_lowrank_set(psg_ctx* ctx, int32_t q, const int32_t* cols, const double* U, const double* B,
+                    const double* Cinv);
 
 /* ---- solve ------------------------------------------------------------------
  * rhs = r2 - A (W r1)  kept on the device (SSP:654-655) */

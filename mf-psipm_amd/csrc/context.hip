@@ -319,7 +319,8 @@ k_pcg_direction(int m, const double* __restrict__ partC, int nbC, const double* 
 __global__ void __launch_bounds__(kBlock)
 k_diag_terms(int m, int nsplit, const int* __restrict__ ptr, const int* __restrict__ idx,
              const double* __restrict__ val, const double* __restrict__ w,
-             const uint8_t* __restrict__ core_mask, double* __restrict__ full, double* __restrict__ core) {
+             const uint8_t* __restrict__ core_mask, const uint8_t* __restrict__ exclude,
+             double* __restrict__ full, double* __restrict__ core) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
     double f = 0.0, c = 0.0;
     // entries of row i in increasing column order across the column blocks
@@ -328,7 +329,7 @@ k_diag_terms(int m, int nsplit, const int* __restrict__ ptr, const int* __restri
       const int j = idx[p];
       const double a = val[p];
       const double t = a * a * w[j];
-      f += t;
+      if (!exclude || !exclude[j]) f += t;
       if (core_mask && core_mask[j]) c += t;
     }
     if (full) full[i] = f;
@@ -463,10 +464,11 @@ static void copy_dev(psg_ctx* c, double* dst, const double* src, int n) {
   HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
 }
 
-void diag_terms_device(psg_ctx* c, const uint8_t* mask_dev, double* full, double* core) {
+void diag_terms_device(psg_ctx* c, const uint8_t* mask_dev, double* full, double* core, bool factor_only) {
   const int grid = grid_for(c->m, kBlock, 4096);
+  const uint8_t* exclude = (factor_only && c->lr_q > 0) ? c->lr_mask.p : nullptr;
   hipLaunchKernelGGL(k_diag_terms, dim3(grid), dim3(kBlock), 0, c->stream, c->m, c->nsplit, c->rptr.p,
-                     c->rcol.p, c->rval.p, c->w.p, mask_dev, full, core);
+                     c->rcol.p, c->rval.p, c->w.p, mask_dev, exclude, full, core);
   HIP_LAUNCH_CHECK();
 }
 
@@ -651,7 +653,7 @@ int psg_diag_terms(psg_ctx* c, const uint8_t* core_mask, double* d_full, double*
       c->mask.upload(core_mask, c->n, c->stream);
       mdev = c->mask.p;
     }
-    diag_terms_device(c, mdev, c->d_full.p, c->d_core.p);
+    diag_terms_device(c, mdev, c->d_full.p, c->d_core.p, false);
     if (d_full) c->d_full.download(d_full, c->m, c->stream);
     if (d_core) c->d_core.download(d_core, c->m, c->stream);
     HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -667,7 +669,7 @@ int psg_scores(psg_ctx* c, const int32_t* candidates, int64_t ncand, double delt
     if (ncand == 0) return PSG_OK;
     for (int64_t k = 0; k < ncand; ++k)
       if (candidates[k] < 0 || candidates[k] >= c->n) throw Error(PSG_ERR_ARGUMENT, "psg_scores: candidate out of range");
-    diag_terms_device(c, nullptr, c->d_full.p, nullptr);
+    diag_terms_device(c, nullptr, c->d_full.p, nullptr, false);
     c->cand.upload(candidates, ncand, c->stream);
     c->cand_out.grow(ncand);
     hipLaunchKernelGGL(k_scores, dim3(grid_for(ncand, kBlock, 8192)), dim3(kBlock), 0, c->stream, (int)ncand,
@@ -1802,3 +1804,114 @@ int psg_ipm_step(psg_ctx* c, int32_t dual, const double* d1, const double* d2, d
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Woodbury correction of the preconditioner for the low-rank columns
+// (_prepare_low_rank_correction / _apply, PS:906-956):
+//   z <- z - B C^-1 U' z,  B = M0^-1 U,  C = Theta_L + U' B  (q <= 8)
+// B and C^-1 are built at factor time by the caller (q preconditioner
+// solves and a q x q Cholesky) and handed over with psg_lowrank_set.
+// ---------------------------------------------------------------------------
+namespace psg {
+
+constexpr int kMaxLowRank = 8;
+
+// partials of U_k . z_e  (k < q, e < NR) at part[(k NR + e) grid + block]
+template <int NR>
+__global__ void __launch_bounds__(kBlock)
+k_lr_dots(int m, int q, const double* __restrict__ U, const double* __restrict__ z, double* __restrict__ part) {
+  double acc[kMaxLowRank * NR];
+#pragma unroll
+  for (int k = 0; k < kMaxLowRank * NR; ++k) acc[k] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    double zi[NR];
+#pragma unroll
+    for (int e = 0; e < NR; ++e) zi[e] = z[i * NR + e];
+#pragma unroll
+    for (int k = 0; k < kMaxLowRank; ++k) {
+      if (k < q) {
+        const double u = U[i + (int64_t)k * m];
+#pragma unroll
+        for (int e = 0; e < NR; ++e) acc[k * NR + e] += u * zi[e];
+      }
+    }
+  }
+  write_partials<kMaxLowRank * NR>(acc, part);
+}
+
+template <int NR>
+__global__ void __launch_bounds__(kBlock)
+k_lr_update(int m, int q, const double* __restrict__ part, int nb, const double* __restrict__ Cinv,
+            const double* __restrict__ B, double* __restrict__ z) {
+  double d[kMaxLowRank * NR];
+  reduce_partials<kMaxLowRank * NR>(part, nb, d);
+  double coef[kMaxLowRank * NR];
+#pragma unroll
+  for (int k = 0; k < kMaxLowRank; ++k)
+#pragma unroll
+    for (int e = 0; e < NR; ++e) {
+      double sacc = 0.0;
+      for (int l = 0; l < q; ++l) sacc += Cinv[k + l * q] * d[l * NR + e];
+      coef[k * NR + e] = (k < q) ? sacc : 0.0;
+    }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int e = 0; e < NR; ++e) {
+      double sacc = 0.0;
+#pragma unroll
+      for (int k = 0; k < kMaxLowRank; ++k)
+        if (k < q) sacc += B[i + (int64_t)k * m] * coef[k * NR + e];
+      z[i * NR + e] -= sacc;
+    }
+  }
+}
+
+void lowrank_correct(psg_ctx* c, double* z, int nr) {
+  if (c->lr_q <= 0 || !c->lr_active) return;
+  const int grid = grid_for(c->m, kBlock, kRedBlocks);
+  if (nr == 1) {
+    hipLaunchKernelGGL(k_lr_dots<1>, dim3(grid), dim3(kBlock), 0, c->stream, c->m, c->lr_q, c->lr_U.p, z, c->lr_part.p);
+    hipLaunchKernelGGL(k_lr_update<1>, dim3(grid), dim3(kBlock), 0, c->stream, c->m, c->lr_q, c->lr_part.p, grid,
+                       c->lr_Cinv.p, c->lr_B.p, z);
+  } else {
+    hipLaunchKernelGGL(k_lr_dots<2>, dim3(grid), dim3(kBlock), 0, c->stream, c->m, c->lr_q, c->lr_U.p, z, c->lr_part.p);
+    hipLaunchKernelGGL(k_lr_update<2>, dim3(grid), dim3(kBlock), 0, c->stream, c->m, c->lr_q, c->lr_part.p, grid,
+                       c->lr_Cinv.p, c->lr_B.p, z);
+  }
+  HIP_LAUNCH_CHECK();
+}
+
+}  // namespace psg
+
+extern "C" int psg_lowrank_set(psg_ctx* c, int32_t q, const int32_t* cols, const double* U, const double* B,
+                               const double* Cinv) {
+  return guarded([&]() -> int {
+    if (!c || q < 0 || q > kMaxLowRank || (q && (!cols || !U)))
+      throw Error(PSG_ERR_ARGUMENT, "psg_lowrank_set: invalid arguments (at most 8 columns)");
+    hipStream_t s = c->stream;
+    if (q == 0) {
+      c->lr_q = 0;
+      c->lr_active = false;
+      return PSG_OK;
+    }
+    std::vector<uint8_t> mask(c->n, 0);
+    for (int k = 0; k < q; ++k) {
+      if (cols[k] < 0 || cols[k] >= c->n) throw Error(PSG_ERR_ARGUMENT, "psg_lowrank_set: column out of range");
+      mask[cols[k]] = 1;
+    }
+    c->lr_mask.upload(mask, s);
+    c->lr_U.upload(U, (size_t)c->m * q, s);
+    c->lr_part.grow((size_t)kMaxLowRank * 2 * kRedBlocks);
+    c->lr_q = q;
+    c->lr_active = false;
+    if (B && Cinv) {
+      for (int k = 0; k < q * q; ++k)
+        if (!std::isfinite(Cinv[k])) throw Error(PSG_ERR_ARGUMENT, "psg_lowrank_set: non-finite core inverse");
+      c->lr_B.upload(B, (size_t)c->m * q, s);
+      c->lr_Cinv.upload(Cinv, (size_t)q * q, s);
+      c->lr_active = true;
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    return PSG_OK;
+  });
+}

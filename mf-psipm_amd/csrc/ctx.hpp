@@ -80,6 +80,13 @@ struct psg_ctx {
   psg::DevBuf<double> ipm_x, ipm_z, ipm_s, ipm_w, ipm_d1, ipm_d2;
   psg::DevBuf<uint8_t> ipm_pos, ipm_box;
   bool have_iterate = false;
+  // Woodbury low-rank columns (PS:906-956): U = A_L (m x q, column-major),
+  // B = M0^-1 U, Cinv = (Theta_L + U' B)^-1; the correction is applied
+  // after every preconditioner solve while lr_active
+  int lr_q = 0;
+  bool lr_active = false;
+  psg::DevBuf<uint8_t> lr_mask;
+  psg::DevBuf<double> lr_U, lr_B, lr_Cinv, lr_part;
 };
 
 namespace psg {
@@ -94,5 +101,8 @@ void factor_apply2(psg_ctx* ctx, const double* r, double* z);
 // nonzero if a dependency wait of the last single-launch solve timed out
 int factor_solve_error(psg_ctx* ctx);
 // context.hip helpers used by factor.hip
-void diag_terms_device(psg_ctx* ctx, const uint8_t* mask_dev, double* full, double* core);
+// factor_only: leave the Woodbury columns out of d_full (the factor's diagonal)
+void diag_terms_device(psg_ctx* ctx, const uint8_t* mask_dev, double* full, double* core, bool factor_only);
+// z -= B C^-1 U' z on NR interleaved columns when a low-rank correction is set
+void lowrank_correct(psg_ctx* ctx, double* z, int nr);
 }  // namespace psg

@@ -1805,7 +1805,9 @@ static int factor_run(psg_ctx* c, const Plan& P, double eta, T* panels, T* dinv)
   hipStream_t s = c->stream;
   // exact dropped diagonal: d_full over all columns, d_core over the active ones
   const int m = c->m;
-  diag_terms_device(c, fs->coremask.p, c->d_full.p, c->d_core.p);
+  // the dropped diagonal covers the factor columns only (SAP:379-392):
+  // Woodbury columns enter through the low-rank correction
+  diag_terms_device(c, fs->coremask.p, c->d_full.p, c->d_core.p, true);
   hipLaunchKernelGGL(k_scaling, dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, c->d_full.p,
                      c->d_core.p, c->delta, fs->dM.p, c->dsc.p);
   HIP_LAUNCH_CHECK();
@@ -2050,12 +2052,12 @@ void factor_apply(psg_ctx* c, const double* r, double* z) {
       apply_run_sf<float>(c, r, z, fs->panels32.p, fs->dinv32.p, fs->vb32.p, fs->vy32.p, fs->vx32.p, fs->sf_part32.p);
     else
       apply_run_sf<double>(c, r, z, fs->panels64.p, fs->dinv64.p, fs->vb64.p, fs->vy64.p, fs->vx64.p, fs->sf_part64.p);
-    return;
-  }
-  if (c->precision == PSG_MIXED)
+  } else if (c->precision == PSG_MIXED) {
     apply_run<float>(c, r, z, fs->panels32.p, fs->dinv32.p, fs->vb32.p, fs->vy32.p, fs->vx32.p, fs->part32.p);
-  else
+  } else {
     apply_run<double>(c, r, z, fs->panels64.p, fs->dinv64.p, fs->vb64.p, fs->vy64.p, fs->vx64.p, fs->part64.p);
+  }
+  lowrank_correct(c, z, 1);
 }
 
 // Two right-hand sides at once (vectors interleaved, entry (i, q) at 2 i + q):
@@ -2067,6 +2069,7 @@ void factor_apply2(psg_ctx* c, const double* r, double* z) {
     apply_run_sf<float, 2>(c, r, z, fs->panels32.p, fs->dinv32.p, fs->vb32.p, fs->vy32.p, fs->vx32.p, fs->sf_part32.p);
   else
     apply_run_sf<double, 2>(c, r, z, fs->panels64.p, fs->dinv64.p, fs->vb64.p, fs->vy64.p, fs->vx64.p, fs->sf_part64.p);
+  lowrank_correct(c, z, 2);
 }
 
 }  // namespace psg

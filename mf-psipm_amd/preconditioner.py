@@ -146,8 +146,6 @@ class HipSpectralStableColumnPreconditioner:
         self.fail_tol = float(fail_tol)
         self.low_rank_columns = low_rank_columns(self.A, low_rank_max_columns, low_rank_min_nnz,
                                                  low_rank_clique_ratio)
-        if self.low_rank_columns.size:
-            raise NotImplementedError("Woodbury low-rank columns are not supported on the GPU path yet")
         mask = np.ones(self.n, dtype=bool)
         mask[self.low_rank_columns] = False
         self.factor_columns = np.flatnonzero(mask)
@@ -267,6 +265,13 @@ class HipSpectralStableColumnPreconditioner:
                    np.ascontiguousarray(self.A.data, dtype=np.float64), prec, C.byref(handle))
         self._ctx = handle
         self._At = None
+        # Woodbury columns: U = A[:, low_rank] dense on the device (PS:906-947)
+        self._lr_cols = np.ascontiguousarray(self.low_rank_columns, dtype=np.int32)
+        self._lr_U = None
+        if self._lr_cols.size:
+            self._lr_U = np.asfortranarray(self.A[:, self.low_rank_columns].toarray())
+            self._call("psg_lowrank_set", self._ctx, self._lr_cols.size, self._lr_cols,
+                       self._lr_U.ravel(order="F"), None, None)
 
     # ------------------------------------------------------------------ utils
     def _call(self, name, *args):
@@ -550,12 +555,46 @@ class HipSpectralStableColumnPreconditioner:
         fms = C.c_double(0.0)
         self._call("psg_last_timings", self._ctx, None, C.byref(fms))
         self.device_factor_ms += fms.value
+        if self._lr_cols.size:
+            try:
+                self._prepare_low_rank(theta)
+            except Exception:
+                self._have_factor = False
+                self.t_fact += time.perf_counter() - t0
+                raise
         self._have_factor = True
         self.nnz_factor = int(nnz.value)
         self.theta = theta
         self.delta = float(delta)
         self.n_fact += 1
         self.t_fact += time.perf_counter() - t0
+
+    def _prepare_low_rank(self, theta):
+        """B = M0^-1 U by q base solves, C = Theta_L + U'B, its Cholesky with
+        the relative shift guard, and C^-1 handed to the device (PS:906-947)."""
+        q = self._lr_cols.size
+        flat_u = self._lr_U.ravel(order="F")
+        self._call("psg_lowrank_set", self._ctx, q, self._lr_cols, flat_u, None, None)  # base solves
+        B = np.empty((self.m, q), order="F")
+        for k in range(q):
+            B[:, k] = self._apply(np.ascontiguousarray(self._lr_U[:, k]))
+        core = self._lr_U.T @ B
+        core[np.diag_indices(q)] += theta[self.low_rank_columns]
+        core = 0.5 * (core + core.T)
+        scale = max(float(np.max(np.abs(np.diag(core)))), 1.0)
+        last = None
+        for rel in (0.0, 1e-14, 1e-12, 1e-10, 1e-8):
+            try:
+                L = np.linalg.cholesky(core if rel == 0.0 else core + rel * scale * np.eye(q))
+            except np.linalg.LinAlgError as exc:
+                last = exc
+                continue
+            Linv = la.solve_triangular(L, np.eye(q), lower=True)
+            cinv = np.asfortranarray(Linv.T @ Linv)
+            self._call("psg_lowrank_set", self._ctx, q, self._lr_cols, flat_u, B.ravel(order="F"),
+                       cinv.ravel(order="F"))
+            return
+        raise self.error_class(f"low-rank correction is not positive definite ({last})")
 
     def _lag_bound(self, weights, delta):
         """Componentwise weight ratio since the last factor (SSP:396-410)."""
@@ -616,7 +655,7 @@ class HipSpectralStableColumnPreconditioner:
         return out
 
     def _apply(self, r):
-        """z = P^-1 r on the GPU (PS:949-956; no Woodbury columns on this path)."""
+        """z = P^-1 r on the GPU, with the Woodbury correction (PS:949-956)."""
         out = np.empty(self.m)
         self._call("psg_apply_P", self._ctx, np.ascontiguousarray(r, dtype=float), out)
         return out
@@ -833,6 +872,97 @@ class HipSpectralStableColumnPreconditioner:
         self.pair_solves += 1
         self.t_solve += max(0.0, time.perf_counter() - started - (self.t_fact - fact_before))
         return out[0], out[1]
+
+
+class HipStableActiveColumnPreconditioner(HipSpectralStableColumnPreconditioner):
+    """The stable (non-spectral) variant: SAP's age-based factorize and its
+    plain PCG solve with one growth retry (psipm_stable_preconditioner.py:
+    396-517), on the same GPU kernels.  It is the object the reference's V3
+    adapter and the stable CLI use.  The Krylov loop is psg_pcg without
+    residual replacement; scipy's criterion |r| <= rtol |rhs| (SAP:450-458)
+    becomes the library's |r| <= tol max(1, |rhs|) with tol = rtol |rhs| /
+    max(1, |rhs|), and a converged recurrence is confirmed by an explicit
+    residual."""
+
+    def __init__(self, A, protected=None, matching_backbone=True, adaptive_refresh_row_threshold=20_000, **kw):
+        kw.setdefault("structural_backbone", bool(matching_backbone))
+        kw.pop("residual_replacement", None)
+        super().__init__(A, protected=protected, residual_replacement=0, **kw)
+        self.adaptive_refresh_row_threshold = max(0, int(adaptive_refresh_row_threshold))
+
+    def set_ipm_context(self, res_d, res_p, res_n, tol):
+        """The stable variant has no Eisenstat-Walker forcing."""
+        self._rhs_index = 0
+
+    def observe(self, x, z, w, s, mu):
+        """mu-based forcing only (SAP:285-291)."""
+        self._ew_tolerance = self.cg_tol_max
+        super().observe(x, z, w, s, mu)
+
+    def factorize(self, theta, delta):
+        """Refresh / refactor by age and Krylov effort (SAP:396-432)."""
+        theta = np.ascontiguousarray(theta, dtype=float)
+        self._weights = 1.0 / theta
+        self._call("psg_set_weights", self._ctx, theta, float(delta))
+        if self.recorder is not None:
+            self._record_trajectory(self._weights, float(delta))
+        poor = self._last_cg >= self.refresh_cg
+        can_grow = self.core_target < self.max_core_target
+        refresh = (not self._initialized or self.core_age >= self.core_max_age
+                   or (poor and (can_grow or self.m <= self.adaptive_refresh_row_threshold)))
+        if refresh:
+            self._refresh_core(self._weights, delta, stable=bool(self.core.size))
+        if not self._have_factor or self.factor_age >= self.factor_max_age or refresh or poor:
+            self._factor_current(theta, float(delta))
+        else:
+            self.theta = theta
+            self.delta = float(delta)
+            self.factor_age += 1
+        self.core_age += 1
+        self.core_sizes.append(int(self.core.size))
+
+    def _stable_pcg(self, r2_norm_rhs):
+        rtol = self.current_cg_tol
+        scale = max(1.0, r2_norm_rhs)
+        res = self._pcg(rtol * r2_norm_rhs / scale if r2_norm_rhs > 0 else rtol)
+        self._record(res)
+        return res
+
+    def solve(self, r1, r2):
+        """SAP.solve (SAP:466-517): PCG, one growth retry, KKTSolveError on failure."""
+        started = time.perf_counter()
+        fact_before = self.t_fact
+        r1 = np.ascontiguousarray(r1, dtype=float)
+        r2 = np.ascontiguousarray(r2, dtype=float)
+        rhs_norm = float(np.linalg.norm(r2 - self.matvec(r1 / self.theta)))
+        self._call("psg_solve_begin", self._ctx, r1, r2)
+        res = self._stable_pcg(rhs_norm)
+        iterations = res.iterations
+        self.cg_tolerances.append(self.current_cg_tol)
+        acceptable = max(self.fail_tol, 2.0 * self.current_cg_tol)
+        failed = res.info != 0 or not np.isfinite(res.relative_residual) or res.relative_residual > acceptable
+        if failed and self.core_target < self.max_core_target:
+            self.pcg_retries += 1
+            self.core_target = min(self.max_core_target,
+                                   max(self.core_target + 1, int(np.ceil(self.retry_growth * self.core_target))))
+            self._refresh_core(self._weights, self.delta, stable=True)
+            self._factor_current(self.theta, self.delta)
+            self.core_sizes.append(int(self.core.size))
+            res = self._stable_pcg(rhs_norm)
+            iterations += res.iterations
+            failed = res.info != 0 or not np.isfinite(res.relative_residual) or res.relative_residual > acceptable
+        self.cg_iterations.append(iterations)
+        self._last_cg = iterations
+        if failed:
+            self.cg_failures += 1
+            self.t_solve += max(0.0, time.perf_counter() - started - (self.t_fact - fact_before))
+            raise self.error_class(f"stable active-column PCG failed: info={res.info}, "
+                                   f"relative residual={res.relative_residual:.2e}, iterations={iterations}")
+        dx, dy = np.empty(self.n), np.empty(self.m)
+        self._call("psg_solve_finish", self._ctx, dx, dy)
+        self.n_solve += 1
+        self.t_solve += max(0.0, time.perf_counter() - started - (self.t_fact - fact_before))
+        return dx, dy
 
 
 def solve_standard(lp, precision="fp64", **kwargs):

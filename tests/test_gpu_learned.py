@@ -115,3 +115,33 @@ def test_select_features_match_numpy(gpu):
                         np.full(cand.size, np.log10(0.37)), np.full(cand.size, float(k))], axis=1)
         assert np.allclose(f, ref, rtol=1e-6, atol=1e-6)
     obj.close()
+
+
+def test_learned_selection_against_the_rule(gpu):
+    """Train on solver trajectories of seeds 1000-1001, evaluate on seed 0.
+
+    Reports what the trained network changes against the rule alone (PCG
+    iterations, factorisations); asserts the same optimum to 1e-8, an exact
+    true residual (status optimal at 1e-8 gap) and a bounded loss (<= 10%
+    more PCG iterations, the fallback's job).
+    """
+    from mf_psipm_amd.selector import collect_trajectories, setcover_problem, solve_with
+    m, n = 600, 20000
+    _say(f"learned vs rule, set cover {m}x{n}")
+    train = [setcover_problem(m, n, s) for s in (1000, 1001)]
+    X, y = collect_trajectories([(lp, tol) for _, lp, tol in train], per_iter=8000)
+    assert X.shape[0] > 1000 and 0.0 < y.mean() < 0.5
+    sel = LearnedColumnSelector(seed=0)
+    losses = sel.fit(X, y, steps=600, batch=2048, lr=2e-3)
+    assert losses[-50:].mean() < losses[:20].mean()
+    raw, lp, tol = setcover_problem(m, n, 0)
+    sol_r, rule = solve_with(lp, tol)
+    sol_l, learned = solve_with(lp, tol, selector=sel)
+    _say(f"rule: pcg {rule['pcg']} factorizations {rule['factorizations']} | learned: pcg {learned['pcg']} "
+         f"factorizations {learned['factorizations']} learned refreshes {learned['learned_refreshes']} "
+         f"fallbacks {learned['learned_fallbacks']}")
+    assert rule["status"] == learned["status"] == "optimal"
+    assert abs(rule["obj"] - learned["obj"]) <= 1e-8 * abs(rule["obj"])
+    assert learned["learned_refreshes"] > 0
+    assert learned["pcg"] <= 1.10 * rule["pcg"]
+    sel.close()

@@ -319,3 +319,80 @@ def test_apply_H_column_blocked_csr(gpu, monkeypatch, nsplit):
     r1, r2 = np.random.default_rng(64).standard_normal(6000), np.random.default_rng(65).standard_normal(2000)
     obj._call("psg_solve_begin", obj._ctx, r1, r2)
     obj.close()
+
+
+def _lp_with_dense_columns(m, n, k, ndense, dense_nnz, seed):
+    """Random feasible LP plus `ndense` columns of `dense_nnz` nonzeros: the
+    reference extracts such columns as Woodbury low-rank updates (PS:691-717)."""
+    from mf_psipm_amd.lp import INF, RawLP
+    rng = np.random.default_rng(seed)
+    raw, *_ = P.generators.random_lp(m, n, k, seed)
+    cols = []
+    for _ in range(ndense):
+        rows = np.sort(rng.choice(m, dense_nnz, replace=False))
+        cols.append(sp.csc_matrix((rng.standard_normal(dense_nnz), (rows, np.zeros(dense_nnz, int))), shape=(m, 1)))
+    A = sp.hstack([raw.A] + cols, format="csc")
+    N = A.shape[1]
+    x0 = rng.uniform(0.1, 1.0, N)
+    y0 = rng.standard_normal(m)
+    z0 = rng.uniform(0.1, 1.0, N)
+    b = A @ x0
+    c = A.T @ y0 + z0
+    return RawLP(c=c, A=A, rl=b.copy(), ru=b.copy(), cl=np.zeros(N), cu=np.full(N, INF), name="dense-cols")
+
+
+def test_woodbury_low_rank_columns_match_oracle(gpu):
+    _say("woodbury low-rank columns m=1500")
+    raw = _lp_with_dense_columns(1500, 4500, 5, 3, 1200, 61)
+    lp = P.lp.to_standard_form(raw)
+    P.lp.scale_problem(lp)
+    g, c = _pair(lp.A, lp.pos, cg_tol=1e-10, cg_tol_max=1e-10)
+    assert g.low_rank_columns.size == 3 and np.array_equal(g.low_rank_columns, c.low_rank_columns)
+    theta = _theta(lp.A.shape[1], 62, spread=2.0)
+    for obj in (g, c):
+        obj.factorize(theta, 1e-6)
+    assert np.array_equal(g.core, c.core)
+    rng = np.random.default_rng(63)
+    for _ in range(3):
+        r = rng.standard_normal(lp.A.shape[0])
+        ref = c.apply(r)
+        assert np.linalg.norm(g._apply(r) - ref) <= 1e-9 * np.linalg.norm(ref)
+    r1, r2 = rng.standard_normal(lp.A.shape[1]), rng.standard_normal(lp.A.shape[0])
+    _, dyg = g.solve(r1, r2)
+    _, dyc = c.solve(r1, r2)
+    assert np.linalg.norm(dyg - dyc) <= 1e-7 * np.linalg.norm(dyc)
+    g.close()
+
+
+def test_woodbury_end_to_end_matches_highs(gpu):
+    _say("woodbury end-to-end m=800")
+    from scipy.optimize import linprog
+    raw = _lp_with_dense_columns(800, 2400, 5, 2, 1100, 71)
+    lp = P.lp.to_standard_form(raw)
+    P.lp.scale_problem(lp)
+    tol = P.lp.scaled_tolerance(lp, 1e-8)
+    solver = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, cg_tol=1e-8, cg_tol_max=1e-8)
+    assert solver.low_rank_columns.size == 2
+    sol = P.ipm.psipm_solve(lp, solver, tol=tol, compl="gap", maxit=160)
+    info = P.lp.recover(lp, raw, sol.x, sol.y, sol.z, sol.w)
+    ref = linprog(raw.c, A_eq=raw.A, b_eq=raw.rl, bounds=(0, None), method="highs-ds")
+    assert sol.status == "optimal"
+    assert abs(info["obj"] - ref.fun) <= 1e-8 * abs(ref.fun)
+    assert max(info["pinf"], info["dinf"]) <= 1e-8
+    solver.close()
+
+
+def test_stable_variant_end_to_end_matches_highs(gpu):
+    _say("stable (non-spectral) variant end-to-end m=1600")
+    from scipy.optimize import linprog
+    from mf_psipm_amd.preconditioner import HipStableActiveColumnPreconditioner
+    raw, lp = _lp(1600, 4800, 5, 2)
+    tol = P.lp.scaled_tolerance(lp, 1e-8)
+    solver = HipStableActiveColumnPreconditioner(lp.A, protected=~lp.pos, cg_tol=1e-8, cg_tol_max=1e-8)
+    sol = P.ipm.psipm_solve(lp, solver, tol=tol, compl="gap", maxit=160)
+    info = P.lp.recover(lp, raw, sol.x, sol.y, sol.z, sol.w)
+    ref = linprog(raw.c, A_eq=raw.A, b_eq=raw.rl, bounds=(0, None), method="highs-ds")
+    assert sol.status == "optimal"
+    assert abs(info["obj"] - ref.fun) <= 1e-8 * abs(ref.fun)
+    assert max(info["pinf"], info["dinf"]) <= 1e-8
+    solver.close()
