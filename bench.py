@@ -70,6 +70,13 @@ def gpu_leg(lp, args, precision):
     t0 = time.perf_counter()
     solver.factorize(theta, 1e-6)
     factor_wall = time.perf_counter() - t0
+    tuned = None
+    if not args.no_autotune:
+        from mf_psipm_amd.autotune import autotune
+
+        t0 = time.perf_counter()
+        tuned = autotune(solver, reps=10, rounds=1)
+        log(f"autotune ({precision}, {time.perf_counter() - t0:.1f}s): {tuned}")
     r1 = np.zeros(n)
     r2 = rng.standard_normal(m)
     solver._call("psg_solve_begin", solver._ctx, r1, r2)
@@ -107,6 +114,7 @@ def gpu_leg(lp, args, precision):
         factor_device_ms=solver.device_factor_ms, core=int(solver.core.size),
         nnz_L=int(solver.nnz_factor), plan=dict(solver._plan.info), hv_ms=kt[0], hv_loop_ms=kt[1],
         apply_ms=kt[2], dot_ms=kt[3], theta=theta, r2=r2, core_cols=solver.core.copy(), pair=pair,
+        tuned=tuned,
     )
     solver.close()
     return out
@@ -254,6 +262,7 @@ def main():
     ap.add_argument("--no-gap", action="store_true")
     ap.add_argument("--cpu-core-factor", type=float, default=0.05)
     ap.add_argument("--no-pair", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-core-factor", type=float, default=0.1)
@@ -350,6 +359,7 @@ def main():
         "pair_column_it_per_s_mixed": None if mixed is None or mixed["pair"] is None
         else round(mixed["pair"]["it_per_s"], 3),
         "batch_64_lps": batch,
+        "autotune": None if g64["tuned"] is None else {k: v[0] for k, v in g64["tuned"].items()},
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["it_per_s"], 4), "unit": "PCG it/s", "cores": 1, "kind": "port",
             "sample": f"{cpu['iterations']} PCG iterations (~{args.cpu_seconds:.0f}s) of the FP64 CPU "

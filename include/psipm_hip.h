@@ -196,6 +196,20 @@ int psg_ipm_observe(psg_ctx* ctx, const double* x, const double* z, const double
  * (PS:1831-1837, 2101-2106; d2 may be NULL when no column is boxed) */
 int psg_ipm_step(psg_ctx* ctx, int32_t dual, const double* d1, const double* d2, double* ratio);
 
+/* ---- launch-shape knobs (autotuner) ------------------------------------------
+ * No counterpart in the reference.  Lanes per column of the A^T gather and
+ * per row of the A gather (powers of two <= 64), and the single-launch
+ * triangular solves' grid, poll backoff cap and tickets per atomic (these
+ * three need a factor).  The results do not depend on the knobs beyond
+ * summation order; autotune.py times them with psg_time_kernels. */
+#define PSG_KNOB_G_CSC 0
+#define PSG_KNOB_G_CSR 1
+#define PSG_KNOB_SF_GRID 2
+#define PSG_KNOB_SF_SLEEP 3
+#define PSG_KNOB_SF_BATCH 4
+int psg_knob_get(psg_ctx* ctx, int32_t knob, int32_t* value);
+int psg_knob_set(psg_ctx* ctx, int32_t knob, int32_t value);
+
 /* ---- timing ---------------------------------------------------------------- */
 /* Device time of the hot-path pieces, measured with HIP events on the
  * library's own stream: out[0] = H v (both gathers, FP64 values),

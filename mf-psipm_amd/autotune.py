@@ -1,0 +1,75 @@
+"""Autotuner of the hot path's launch shapes.
+
+The SpMV gathers assign a group of G lanes to each column (A^T v) and each
+row (A t); the single-launch triangular solves run a persistent grid whose
+size, poll backoff and ticket batch decide how well the dependency chain
+overlaps.  The best values depend on the matrix (nonzeros per column/row)
+and on the factor (supernode count, tree depth), so they are measured, not
+guessed: coordinate descent over each knob's candidate values, timing the
+affected kernels with HIP events on the library's own stream
+(``psg_time_kernels``), keeping a value only when it beats the incumbent by
+``margin``.  Results are exact-arithmetic neutral: only summation order
+changes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+KNOB_G_CSC, KNOB_G_CSR, KNOB_SF_GRID, KNOB_SF_SLEEP, KNOB_SF_BATCH = range(5)
+NAMES = {KNOB_G_CSC: "g_csc", KNOB_G_CSR: "g_csr", KNOB_SF_GRID: "sf_grid", KNOB_SF_SLEEP: "sf_sleep",
+         KNOB_SF_BATCH: "sf_batch"}
+SPACE = {
+    KNOB_G_CSC: (1, 2, 4, 8, 16, 32),
+    KNOB_G_CSR: (2, 4, 8, 16, 32, 64),
+    KNOB_SF_GRID: (256, 512, 1024, 2048, 4096),
+    KNOB_SF_SLEEP: (1, 4, 16, 64),
+    KNOB_SF_BATCH: (1, 2, 4),
+}
+# psg_time_kernels slot each knob is judged by: 1 = H v in the loop's
+# precision, 2 = P^-1 r
+SLOT = {KNOB_G_CSC: 1, KNOB_G_CSR: 1, KNOB_SF_GRID: 2, KNOB_SF_SLEEP: 2, KNOB_SF_BATCH: 2}
+
+
+def get(obj, knob) -> int:
+    v = C.c_int32()
+    obj._call("psg_knob_get", obj._ctx, int(knob), C.byref(v))
+    return int(v.value)
+
+
+def set_(obj, knob, value) -> None:
+    obj._call("psg_knob_set", obj._ctx, int(knob), int(value))
+
+
+def _time(obj, slot, reps):
+    out = np.zeros(4)
+    obj._call("psg_time_kernels", obj._ctx, int(reps), out)
+    return float(out[slot])
+
+
+def autotune(obj, reps=10, rounds=2, margin=0.02, knobs=None):
+    """Tune the knobs of a factorised GPU object in place.
+
+    Returns {name: (chosen value, ms before, ms after)}.  ``obj`` is a
+    HipSpectralStableColumnPreconditioner (or anything with ``_call`` and
+    ``_ctx``) whose weights and factor are set.
+    """
+    knobs = list(SPACE) if knobs is None else list(knobs)
+    report = {}
+    for _ in range(max(1, int(rounds))):
+        for k in knobs:
+            slot = SLOT[k]
+            best_v = get(obj, k)
+            before = best_t = _time(obj, slot, reps)
+            for v in SPACE[k]:
+                if v == best_v:
+                    continue
+                set_(obj, k, v)
+                t = _time(obj, slot, reps)
+                if t < best_t * (1.0 - margin):
+                    best_v, best_t = v, t
+            set_(obj, k, best_v)
+            prev = report.get(NAMES[k])
+            report[NAMES[k]] = (best_v, prev[1] if prev else before, best_t)
+    return report

@@ -1915,3 +1915,40 @@ extern "C" int psg_lowrank_set(psg_ctx* c, int32_t q, const int32_t* cols, const
     return PSG_OK;
   });
 }
+
+// ---------------------------------------------------------------------------
+// Launch-shape knobs read and written by the autotuner (autotune.py)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int psg_knob_get(psg_ctx* c, int32_t knob, int32_t* value) {
+  return guarded([&]() -> int {
+    if (!c || !value) throw Error(PSG_ERR_ARGUMENT, "psg_knob_get: null argument");
+    if (knob == PSG_KNOB_G_CSC) { *value = c->g_csc; return PSG_OK; }
+    if (knob == PSG_KNOB_G_CSR) { *value = c->g_csr; return PSG_OK; }
+    int* p = factor_knob(c, knob);
+    if (!p) throw Error(knob >= 2 && knob <= 4 ? PSG_ERR_STATE : PSG_ERR_ARGUMENT, "psg_knob_get: unknown knob or no factor");
+    *value = *p;
+    return PSG_OK;
+  });
+}
+
+int psg_knob_set(psg_ctx* c, int32_t knob, int32_t value) {
+  return guarded([&]() -> int {
+    if (!c) throw Error(PSG_ERR_ARGUMENT, "psg_knob_set: null context");
+    auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+    if (knob == PSG_KNOB_G_CSC || knob == PSG_KNOB_G_CSR) {
+      if (!pow2(value) || value > 64) throw Error(PSG_ERR_ARGUMENT, "psg_knob_set: group size must be 1, 2, 4, ..., 64");
+      (knob == PSG_KNOB_G_CSC ? c->g_csc : c->g_csr) = value;
+      return PSG_OK;
+    }
+    int* p = factor_knob(c, knob);
+    if (!p) throw Error(knob >= 2 && knob <= 4 ? PSG_ERR_STATE : PSG_ERR_ARGUMENT, "psg_knob_set: unknown knob or no factor");
+    const int lo = 1, hi = knob == PSG_KNOB_SF_GRID ? 65536 : (knob == PSG_KNOB_SF_SLEEP ? 1024 : 64);
+    if (value < lo || value > hi) throw Error(PSG_ERR_ARGUMENT, "psg_knob_set: value out of range");
+    *p = value;
+    return PSG_OK;
+  });
+}
+
+}  // extern "C"

@@ -98,6 +98,9 @@ bool factor_ready(const psg_ctx* ctx);
 void factor_apply(psg_ctx* ctx, const double* r, double* z);
 // the same for two right-hand sides, interleaved (entry (i, q) at 2 i + q)
 void factor_apply2(psg_ctx* ctx, const double* r, double* z);
+// launch-shape knob of the factor's solve (2 grid, 3 poll backoff cap,
+// 4 tickets per atomic); nullptr without a factor
+int* factor_knob(psg_ctx* ctx, int knob);
 // nonzero if a dependency wait of the last single-launch solve timed out
 int factor_solve_error(psg_ctx* ctx);
 // context.hip helpers used by factor.hip

@@ -1999,6 +1999,17 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   HIP_LAUNCH_CHECK();
 }
 
+int* factor_knob(psg_ctx* c, int knob) {
+  FactorState* fs = c->fs;
+  if (!fs || !fs->ready) return nullptr;
+  switch (knob) {
+    case 2: return &fs->sf_grid;
+    case 3: return &fs->sf_sleep;
+    case 4: return &fs->sf_batch;
+    default: return nullptr;
+  }
+}
+
 int factor_solve_error(psg_ctx* c) {
   FactorState* fs = c->fs;
   if (!fs || fs->solve_mode == 0 || !fs->flags.p) return 0;

@@ -365,9 +365,9 @@ def test_woodbury_low_rank_columns_match_oracle(gpu):
 
 
 def test_woodbury_end_to_end_matches_highs(gpu):
-    _say("woodbury end-to-end m=800")
+    _say("woodbury end-to-end m=1200")
     from scipy.optimize import linprog
-    raw = _lp_with_dense_columns(800, 2400, 5, 2, 1100, 71)
+    raw = _lp_with_dense_columns(1200, 3600, 5, 2, 1100, 71)
     lp = P.lp.to_standard_form(raw)
     P.lp.scale_problem(lp)
     tol = P.lp.scaled_tolerance(lp, 1e-8)
@@ -395,4 +395,52 @@ def test_stable_variant_end_to_end_matches_highs(gpu):
     assert sol.status == "optimal"
     assert abs(info["obj"] - ref.fun) <= 1e-8 * abs(ref.fun)
     assert max(info["pinf"], info["dinf"]) <= 1e-8
+    solver.close()
+
+
+def test_launch_knobs_change_speed_not_results(gpu):
+    _say("launch knobs and autotune m=1600")
+    from mf_psipm_amd import autotune as AT
+    _, lp = _lp(1600, 4800, 5, 81)
+    obj = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, structural_backbone=False, core_factor=0.5)
+    obj.factorize(_theta(4800, 82, 2.0), 1e-5)
+    rng = np.random.default_rng(83)
+    v = rng.standard_normal(1600)
+    h0, z0 = obj._full_matvec(v), obj._apply(v)
+    for knob, value in ((AT.KNOB_G_CSC, 1), (AT.KNOB_G_CSR, 64), (AT.KNOB_SF_GRID, 256), (AT.KNOB_SF_SLEEP, 1),
+                        (AT.KNOB_SF_BATCH, 4)):
+        AT.set_(obj, knob, value)
+        assert AT.get(obj, knob) == value
+    assert np.allclose(obj._full_matvec(v), h0, rtol=1e-13, atol=1e-13 * np.abs(h0).max())
+    assert np.linalg.norm(obj._apply(v) - z0) <= 1e-12 * np.linalg.norm(z0)
+    with pytest.raises(_lib.LibraryError):
+        AT.set_(obj, AT.KNOB_G_CSC, 3)
+    with pytest.raises(_lib.LibraryError):
+        AT.set_(obj, AT.KNOB_SF_GRID, 0)
+    report = AT.autotune(obj, reps=3, rounds=1)
+    assert set(report) == set(AT.NAMES.values())
+    assert all(after <= before * 1.05 for _, before, after in report.values())
+    assert np.linalg.norm(obj._apply(v) - z0) <= 1e-12 * np.linalg.norm(z0)
+    obj.close()
+
+
+def test_network_grid_matches_highs(gpu):
+    """Min-cost flow on a 40 x 40 grid through the network pipeline of
+    batch_test_vlsi_spectral.py:242-346 (grounding, power-of-two cost
+    scaling, no matching backbone)."""
+    _say("network grid 40x40")
+    from scipy.optimize import linprog
+    full = P.generators.grid_min_cost_flow(40, 40, 60, 3)
+    lp, _ = P.lp.ground_network(full)
+    c0 = lp.c.copy()
+    osc = float(2.0 ** np.ceil(np.log2(np.abs(c0).max())))
+    lp.c = c0 / osc
+    solver = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, structural_backbone=False,
+                                                   cg_tol=1e-8, cg_tol_max=1e-8)
+    sol = P.ipm.psipm_solve(lp, solver, tol=1e-8, compl="gap", maxit=200)
+    ref = linprog(c0, A_eq=lp.A, b_eq=lp.b, bounds=(0, None), method="highs-ds")
+    assert sol.status == "optimal"
+    assert abs(sol.obj * osc - ref.fun) <= 1e-8 * abs(ref.fun)
+    # the grounded rows are redundant: the full balance holds too
+    assert np.abs(full.b - full.A @ sol.x).max() <= 1e-8 * (1.0 + np.abs(full.b).max())
     solver.close()
