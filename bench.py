@@ -317,6 +317,15 @@ def main():
                           not args.no_sequential)
         log(f"batch: {batch}")
 
+    # measured HBM traffic per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    # calibrated), when a summary exists for this exact workload and factor
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", "traffic_random_lp_200k_x_1m.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            t = json.load(fh)
+        if t.get("nnz_L") == g64["nnz_L"] and (m, n) == (200_000, 1_000_000) and args.seed == 0:
+            traffic = t
     bytes_hv = hv_bytes(m, n, nnz, 8)
     achieved = bytes_hv / (g64["hv_ms"] * 1e-3) / 1e9
     # dominant kernel pair: the two single-launch triangular solves read every
@@ -347,11 +356,12 @@ def main():
                       "apply_P_mixed": None if mixed is None else mixed["apply_ms"]},
         "roofline": {"bound": "hbm", "kernel": "P^-1 r: k_fwd_low + k_fwd_sf + k_tail_lower + k_tail_upper + k_bwd_sf + k_bwd_low (triangular solves)",
                      "achieved": round(achieved_apply, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_apply / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved_apply / HBM_PEAK_GBS, 4), "traffic": traffic.get("apply_P_bytes"),
                      "bytes_per_launch": int(bytes_apply), "ms_per_launch": round(g64["apply_ms"], 4)},
         "roofline_hv": {"bound": "hbm", "kernel": "H v: k_seg_gather + k_csr (+finalize)",
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": int(bytes_hv),
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get("Hv_bytes"),
+                        "bytes_per_launch": int(bytes_hv),
                         "ms_per_launch": round(g64["hv_ms"], 4)},
         "time_to_gap_2000x6000": gaps or None,
         "time_to_gap_setcover_2000x100000": learned,

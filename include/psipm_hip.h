@@ -196,6 +196,13 @@ int psg_ipm_observe(psg_ctx* ctx, const double* x, const double* z, const double
  * (PS:1831-1837, 2101-2106; d2 may be NULL when no column is boxed) */
 int psg_ipm_step(psg_ctx* ctx, int32_t dual, const double* d1, const double* d2, double* ratio);
 
+/* ---- pricing ------------------------------------------------------------------
+ * Column pricing of the active-pool loops (PS:3873-3880, _top_priced
+ * PS:2827-2836): violation_j = ((A^T y)_j - c_j) / scale_j for all columns
+ * (scale NULL: 1).  The caller selects the largest violations. */
+int psg_price_setup(psg_ctx* ctx, const double* cost, const double* scale);
+int psg_price(psg_ctx* ctx, const double* y, double* violation);
+
 /* ---- launch-shape knobs (autotuner) ------------------------------------------
  * No counterpart in the reference.  Lanes per column of the A^T gather and
  * per row of the A gather (powers of two <= 64), and the single-launch

@@ -102,6 +102,8 @@ SIGNATURES = [
     ("psg_batch_pcg", C.c_int, [C.c_void_p, _f64p, _f64p, C.c_int32, C.c_int32, _f64p, _i32p, _i32p, _f64p]),
     ("psg_batch_last", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     ("psg_spmv", C.c_int, [C.c_void_p, C.c_int32, _f64p, _f64p]),
+    ("psg_price_setup", C.c_int, [C.c_void_p, _f64p, _opt(_f64p)]),
+    ("psg_price", C.c_int, [C.c_void_p, _f64p, _f64p]),
     ("psg_knob_get", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     ("psg_knob_set", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     ("psg_lowrank_set", C.c_int, [C.c_void_p, C.c_int32, _opt(_i32p), _opt(_f64p), _opt(_f64p), _opt(_f64p)]),

@@ -1952,3 +1952,43 @@ int psg_knob_set(psg_ctx* c, int32_t knob, int32_t value) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Pricing (PS:3873-3880): violation_j = ((A' y)_j - c_j) / scale_j, the
+// column gather with the cost and the scale fused into its epilogue.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int psg_price_setup(psg_ctx* c, const double* cost, const double* scale) {
+  return guarded([&]() -> int {
+    if (!c || !cost) throw Error(PSG_ERR_ARGUMENT, "psg_price_setup: null argument");
+    std::vector<double> negc(c->n), isc(c->n);
+    for (int j = 0; j < c->n; ++j) {
+      negc[j] = -cost[j];
+      const double sj = scale ? scale[j] : 1.0;
+      if (!(sj > 0.0) || !std::isfinite(sj)) throw Error(PSG_ERR_ARGUMENT, "psg_price_setup: scales must be positive");
+      isc[j] = 1.0 / sj;
+    }
+    c->price_negc.upload(negc, c->stream);
+    c->price_iscale.upload(isc, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->have_price = true;
+    return PSG_OK;
+  });
+}
+
+int psg_price(psg_ctx* c, const double* y, double* violation) {
+  return guarded([&]() -> int {
+    if (!c || !y || !violation) throw Error(PSG_ERR_ARGUMENT, "psg_price: null argument");
+    if (!c->have_price) throw Error(PSG_ERR_STATE, "psg_price: call psg_price_setup first");
+    hipStream_t s = c->stream;
+    c->vin.upload(y, c->m, s);
+    launch_gather<double>(c->g_csc, c->n, c->cptr.p, c->crow.p, c->cval.p, c->vin.p, c->price_iscale.p,
+                          c->price_negc.p, c->dx_n.p, s);
+    c->dx_n.download(violation, c->n, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    return PSG_OK;
+  });
+}
+
+}  // extern "C"

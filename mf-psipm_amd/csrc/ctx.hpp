@@ -83,6 +83,9 @@ struct psg_ctx {
   // Woodbury low-rank columns (PS:906-956): U = A_L (m x q, column-major),
   // B = M0^-1 U, Cinv = (Theta_L + U' B)^-1; the correction is applied
   // after every preconditioner solve while lr_active
+  // pricing: -c and 1/scale of the columns (psg_price_setup)
+  psg::DevBuf<double> price_negc, price_iscale;
+  bool have_price = false;
   int lr_q = 0;
   bool lr_active = false;
   psg::DevBuf<uint8_t> lr_mask;

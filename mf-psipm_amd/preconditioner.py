@@ -355,6 +355,27 @@ class HipSpectralStableColumnPreconditioner:
         self._call("psg_spmv", self._ctx, 1, np.ascontiguousarray(y, dtype=float), out)
         return out
 
+    # ---------------------------------------------------------------- pricing
+    def price(self, y, c, scale=None, mask=None, count=None, threshold=None):
+        """Column pricing of the active-pool loops on the GPU (PS:3873-3880):
+        violation = (A'y - c) / scale; with ``count``, the indices of the
+        largest violations under ``mask`` (and above ``threshold``), sorted
+        descending as _top_priced (PS:2827-2836).  Returns (violation, indices)."""
+        self._call("psg_price_setup", self._ctx, np.ascontiguousarray(c, dtype=float),
+                   None if scale is None else np.ascontiguousarray(scale, dtype=float))
+        v = np.empty(self.n)
+        self._call("psg_price", self._ctx, np.ascontiguousarray(y, dtype=float), v)
+        if count is None:
+            return v, None
+        sel = np.ones(self.n, dtype=bool) if mask is None else np.asarray(mask, dtype=bool).copy()
+        if threshold is not None:
+            sel &= v > threshold
+        idx = np.flatnonzero(sel)
+        k = min(int(count), idx.size)
+        if k < idx.size:
+            idx = idx[np.argpartition(v[idx], -k)[-k:]]
+        return v, idx[np.argsort(v[idx])[::-1]]
+
     def _linear_tolerance(self):
         tol = self.current_cg_tol
         if self._rhs_index:

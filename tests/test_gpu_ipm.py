@@ -62,3 +62,22 @@ def test_step_to_boundary_matches_host(gpu, seed):
         assert obj.ipm_step(0, dx) == ref_p
         assert obj.ipm_step(1, dz, dw) == ref_d
     obj.close()
+
+
+def test_pricing_matches_host_top_priced(gpu):
+    _say("pricing A'y - c over scale, top 50")
+    lp, obj = _obj(seed=4)
+    n = lp.A.shape[1]
+    rng = np.random.default_rng(7)
+    y = rng.standard_normal(lp.A.shape[0])
+    c = rng.standard_normal(n)
+    scale = 10.0 ** rng.uniform(-1, 1, n)
+    mask = rng.uniform(size=n) < 0.7
+    v, idx = obj.price(y, c, scale=scale, mask=mask, count=50, threshold=0.0)
+    ref = (lp.A.T @ y - c) / scale
+    assert np.allclose(v, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+    # _top_priced (PS:2827-2836) on the host values
+    cand = np.flatnonzero(mask & (ref > 0.0))
+    top = cand[np.argsort(ref[cand])[::-1][:50]]
+    assert np.array_equal(idx, top)
+    obj.close()
