@@ -207,13 +207,17 @@ int psg_price(psg_ctx* ctx, const double* y, double* violation);
  * No counterpart in the reference.  Lanes per column of the A^T gather and
  * per row of the A gather (powers of two <= 64), and the single-launch
  * triangular solves' grid, poll backoff cap and tickets per atomic (these
- * three need a factor).  The results do not depend on the knobs beyond
+ * three need a factor), and the PCG loop's launch batch.  The results do not depend on the knobs beyond
  * summation order; autotune.py times them with psg_time_kernels. */
 #define PSG_KNOB_G_CSC 0
 #define PSG_KNOB_G_CSR 1
 #define PSG_KNOB_SF_GRID 2
 #define PSG_KNOB_SF_SLEEP 3
 #define PSG_KNOB_SF_BATCH 4
+/* PCG iterations launched between host synchronisations (1..64, default 4);
+ * a device-side judge holds the loop wherever the host must act, so the
+ * iterates do not depend on it */
+#define PSG_KNOB_PCG_BATCH 5
 int psg_knob_get(psg_ctx* ctx, int32_t knob, int32_t* value);
 int psg_knob_set(psg_ctx* ctx, int32_t knob, int32_t value);
 

@@ -315,6 +315,106 @@ k_pcg_direction(int m, const double* __restrict__ partC, int nbC, const double* 
   }
 }
 
+// ---- device-judged PCG loop: the host launches several iterations between
+// synchronisations; a judge kernel raises kHold when the host must act (an
+// explicit residual check, a replacement, the iteration limit) and every
+// vector update is a no-op while kHold or kStop is set.  The arithmetic and
+// its order are those of the host-judged loop.
+
+// alpha = rz / (p.q) with the guards of SSP:490-501, x += alpha p, r -= alpha q
+__global__ void __launch_bounds__(kBlock)
+k_pcg_step_d(int m, const double* __restrict__ hv_dots, const double* __restrict__ rzs,
+             const double* __restrict__ p, const double* __restrict__ q, double* __restrict__ x,
+             double* __restrict__ r, double* __restrict__ scal, double* __restrict__ alphas) {
+  if (scal[kStop] != 0.0 || scal[kHold] != 0.0) return;
+  const double curvature = hv_dots[0];
+  const double floor_ = std::numeric_limits<double>::min() * fmax(1.0, sqrt(hv_dots[1]) * sqrt(hv_dots[2]));
+  const double rz = rzs[(int)scal[kRzSlot]];
+  const double alpha = rz / curvature;
+  int fail = 0;
+  if (!isfinite(curvature) || curvature <= floor_) fail = -2;
+  else if (!isfinite(alpha) || alpha <= 0.0) fail = -3;
+  if (fail) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      scal[kInfo] = fail;
+      scal[kCurv] = curvature;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[kStop] = 1.0;
+    return;
+  }
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    acc[0] += ri * ri;
+  }
+  block_sum<1>(acc);
+  if (threadIdx.x == 0) {
+    scal[kScalarHeader + blockIdx.x] = acc[0];
+    if (blockIdx.x == 0) {
+      scal[kAlpha] = alpha;
+      scal[kCurv] = curvature;
+      scal[kRzUsed] = rz;
+      alphas[(int)scal[kIt]] = alpha;
+    }
+  }
+}
+
+// one thread: count the step, keep |r|^2 and hold for the host when the
+// recurrence meets the tolerance, a replacement is due or maxit is reached
+__global__ void k_pcg_judge(int nb, double* __restrict__ scal) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  if (scal[kStop] != 0.0 || scal[kHold] != 0.0) return;
+  double rr = 0.0;
+  for (int b = 0; b < nb; ++b) rr += scal[kScalarHeader + b];  // the host's fixed order
+  const int it = (int)scal[kIt] + 1;
+  scal[kIt] = it;
+  scal[kRR] = rr;
+  const int repl = (int)scal[kRepl];
+  const double rec = sqrt(rr) / scal[kScale];
+  if ((repl > 0 && it % repl == 0) || rec <= scal[kTol] || it >= (int)scal[kMaxit]) scal[kHold] = 1.0;
+}
+
+// beta = rz_new / rz (guards of SSP:545-553), p = z + beta p
+__global__ void __launch_bounds__(kBlock)
+k_pcg_direction_d(int m, const double* __restrict__ partC, int nbC, double* __restrict__ rzs,
+                  const double* __restrict__ z, double* __restrict__ p, double* __restrict__ scal,
+                  double* __restrict__ betas) {
+  if (scal[kStop] != 0.0 || scal[kHold] != 0.0) return;
+  double tot[2];
+  reduce_partials<2>(partC, nbC, tot);
+  const double rz_new = tot[0];
+  const double beta = rz_new / scal[kRzUsed];
+  int fail = 0;
+  if (!isfinite(rz_new) || rz_new <= 0.0) fail = -6;
+  else if (!isfinite(beta) || beta < 0.0) fail = -7;
+  if (fail) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      scal[kInfo] = fail;
+      scal[kStop] = 1.0;
+    }
+    return;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int it = (int)scal[kIt];
+    rzs[it] = rz_new;
+    scal[kRzSlot] = it;
+    const int nbeta = (int)scal[kNBeta];
+    betas[nbeta] = beta;
+    scal[kNBeta] = nbeta + 1;
+    scal[kBeta] = beta;
+  }
+}
+
+// set slot k of the scalar block (kernel argument, no host buffer involved)
+__global__ void k_scal_set(double* __restrict__ scal, int k, double v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[k] = v;
+}
+
 // d_full_i = sum_j a_ij^2 w_j [full_mask_j], d_core_i = same over core_mask (SAP:379-390)
 __global__ void __launch_bounds__(kBlock)
 k_diag_terms(int m, int nsplit, const int* __restrict__ ptr, const int* __restrict__ idx,
@@ -741,30 +841,57 @@ int psg_pcg(psg_ctx* c, double tol, int32_t maxit, int32_t replacement, double g
     nb = device_dot(c, c->r.p, c->z.p, m, c->partD.p);
     read_partials(c, c->partD.p, nb, 1, tmp);
     double rz = tmp[0];
-    HIP_CHECK(hipMemcpyAsync(c->rzs.p, &rz, sizeof(double), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_scal_set, dim3(1), dim3(1), 0, s, c->rzs.p, 0, rz);
+    HIP_LAUNCH_CHECK();
     copy_dev(c, c->p.p, c->z.p, m);
     int info = maxit;
     if (hist.back() <= tol) info = 0;
     else if (!std::isfinite(rz) || rz <= 0.0) info = -1;
-    int it = 0, rz_slot = 0, alpha_start = 0, beta_start = 0;
+    int it = 0, alpha_start = 0, beta_start = 0;
     int replacements = 0, checks = 0;
     const int grid_m = grid_for(m, kBlock, kRedBlocks);
-    while (info > 0 && it < maxit) {
-      device_Hv(c, c->p.p, c->q.p, c->partA.p, true, c->hvdots.p);
-      hipLaunchKernelGGL(k_pcg_step, dim3(grid_m), dim3(kBlock), 0, s, m, c->hvdots.p, c->rzs.p + rz_slot,
-                         c->p.p, c->q.p, c->x.p, c->r.p, c->scal.p, c->alphas.p + it);
+    auto sset = [&](int k, double v) {
+      hipLaunchKernelGGL(k_scal_set, dim3(1), dim3(1), 0, s, c->scal.p, k, v);
       HIP_LAUNCH_CHECK();
-      HIP_CHECK(hipMemcpyAsync(c->hscal, c->scal.p, (kScalarHeader + grid_m) * sizeof(double),
-                               hipMemcpyDeviceToHost, s));
+    };
+    sset(kIt, 0.0);
+    sset(kHold, 0.0);
+    sset(kRzSlot, 0.0);
+    sset(kScale, scale);
+    sset(kTol, tol);
+    sset(kRepl, (double)replacement);
+    sset(kMaxit, (double)maxit);
+    // one iteration: H p, step, judge, P^-1 r, r.z, direction (no-ops under kHold / kStop)
+    auto iteration = [&]() {
+      device_Hv(c, c->p.p, c->q.p, c->partA.p, true, c->hvdots.p);
+      hipLaunchKernelGGL(k_pcg_step_d, dim3(grid_m), dim3(kBlock), 0, s, m, c->hvdots.p, c->rzs.p, c->p.p, c->q.p,
+                         c->x.p, c->r.p, c->scal.p, c->alphas.p);
+      hipLaunchKernelGGL(k_pcg_judge, dim3(1), dim3(1), 0, s, grid_m, c->scal.p);
+      factor_apply(c, c->r.p, c->z.p);
+      const int nbC = device_dot(c, c->r.p, c->z.p, m, c->partC.p);
+      hipLaunchKernelGGL(k_pcg_direction_d, dim3(grid_m), dim3(kBlock), 0, s, m, c->partC.p, nbC, c->rzs.p, c->z.p,
+                         c->p.p, c->scal.p, c->betas.p);
+      HIP_LAUNCH_CHECK();
+    };
+    const int batch = std::max(1, c->pcg_batch);
+    while (info > 0 && it < maxit) {
+      // iterations until the next replacement boundary or the limit: the
+      // judge holds there anyway, so a batch never runs past it
+      int k = std::min(batch, maxit - it);
+      if (replacement > 0) k = std::min(k, replacement - it % replacement);
+      for (int q = 0; q < k; ++q) iteration();
+      HIP_CHECK(hipMemcpyAsync(c->hscal, c->scal.p, kScalarHeader * sizeof(double), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
+      it = (int)c->hscal[kIt];
       if (c->hscal[kStop] != 0.0) {
         info = (int)c->hscal[kInfo];
         break;
       }
-      ++it;
-      const double rr = host_sum(c->hscal + kScalarHeader, grid_m);
-      const double rec = std::sqrt(rr) / scale;
+      if (c->hscal[kHold] == 0.0) continue;
+      // the judge held after the step of iteration `it`: act as the host-judged loop
+      const double rec = std::sqrt(c->hscal[kRR]) / scale;
       const bool replace = replacement > 0 && it % replacement == 0;
+      bool reset = false;
       if (replace || rec <= tol) {
         nb = device_residual(c, c->rhs.p, c->x.p, c->r.p, c->rt.p, c->partD.p);
         read_partials(c, c->partD.p, nb, 2, tmp);
@@ -785,22 +912,27 @@ int psg_pcg(psg_ctx* c, double tol, int32_t maxit, int32_t replacement, double g
           factor_apply(c, c->r.p, c->z.p);
           nb = device_dot(c, c->r.p, c->z.p, m, c->partD.p);
           read_partials(c, c->partD.p, nb, 1, tmp);
-          rz = tmp[0];
+          const double rz = tmp[0];
           if (!std::isfinite(rz) || rz <= 0.0) { info = -5; break; }
-          HIP_CHECK(hipMemcpyAsync(c->rzs.p + it, &rz, sizeof(double), hipMemcpyHostToDevice, s));
-          rz_slot = it;
+          hipLaunchKernelGGL(k_scal_set, dim3(1), dim3(1), 0, s, c->rzs.p, it, rz);
+          HIP_LAUNCH_CHECK();
+          sset(kRzSlot, (double)it);
           copy_dev(c, c->p.p, c->z.p, m);
           alpha_start = it;
           beta_start = (int)c->hscal[kNBeta];
-          continue;
+          reset = true;
         }
       }
-      factor_apply(c, c->r.p, c->z.p);
-      const int nbC = device_dot(c, c->r.p, c->z.p, m, c->partC.p);
-      hipLaunchKernelGGL(k_pcg_direction, dim3(grid_m), dim3(kBlock), 0, s, m, c->partC.p, nbC,
-                         c->rzs.p + rz_slot, c->rzs.p + it, c->z.p, c->p.p, c->scal.p, c->betas.p);
-      HIP_LAUNCH_CHECK();
-      rz_slot = it;
+      if (it >= maxit) break;
+      sset(kHold, 0.0);
+      if (!reset) {
+        // the direction update the hold skipped
+        factor_apply(c, c->r.p, c->z.p);
+        const int nbC = device_dot(c, c->r.p, c->z.p, m, c->partC.p);
+        hipLaunchKernelGGL(k_pcg_direction_d, dim3(grid_m), dim3(kBlock), 0, s, m, c->partC.p, nbC, c->rzs.p, c->z.p,
+                           c->p.p, c->scal.p, c->betas.p);
+        HIP_LAUNCH_CHECK();
+      }
     }
     // pick up a direction-kernel failure raised after the last synchronisation
     HIP_CHECK(hipMemcpyAsync(c->hscal, c->scal.p, kScalarHeader * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1925,6 +2057,7 @@ int psg_knob_get(psg_ctx* c, int32_t knob, int32_t* value) {
   return guarded([&]() -> int {
     if (!c || !value) throw Error(PSG_ERR_ARGUMENT, "psg_knob_get: null argument");
     if (knob == PSG_KNOB_G_CSC) { *value = c->g_csc; return PSG_OK; }
+    if (knob == PSG_KNOB_PCG_BATCH) { *value = c->pcg_batch; return PSG_OK; }
     if (knob == PSG_KNOB_G_CSR) { *value = c->g_csr; return PSG_OK; }
     int* p = factor_knob(c, knob);
     if (!p) throw Error(knob >= 2 && knob <= 4 ? PSG_ERR_STATE : PSG_ERR_ARGUMENT, "psg_knob_get: unknown knob or no factor");
@@ -1937,6 +2070,11 @@ int psg_knob_set(psg_ctx* c, int32_t knob, int32_t value) {
   return guarded([&]() -> int {
     if (!c) throw Error(PSG_ERR_ARGUMENT, "psg_knob_set: null context");
     auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+    if (knob == PSG_KNOB_PCG_BATCH) {
+      if (value < 1 || value > 64) throw Error(PSG_ERR_ARGUMENT, "psg_knob_set: PCG batch must lie in [1, 64]");
+      c->pcg_batch = value;
+      return PSG_OK;
+    }
     if (knob == PSG_KNOB_G_CSC || knob == PSG_KNOB_G_CSR) {
       if (!pow2(value) || value > 64) throw Error(PSG_ERR_ARGUMENT, "psg_knob_set: group size must be 1, 2, 4, ..., 64");
       (knob == PSG_KNOB_G_CSC ? c->g_csc : c->g_csr) = value;

@@ -15,6 +15,15 @@ enum ScalarSlot {
   kCurv = 3,
   kNBeta = 4,    // number of accepted beta values
   kBeta = 5,
+  kIt = 6,       // iterations taken (device-judged loop)
+  kHold = 7,     // nonzero: the host must act before the loop goes on
+  kRR = 8,       // |r|^2 of the last recurrence
+  kRzSlot = 9,   // rzs index holding the current r.z
+  kRzUsed = 10,  // r.z the last step used (read by the direction kernel)
+  kScale = 11,   // max(1, |rhs|)
+  kTol = 12,
+  kRepl = 13,    // residual replacement period (0: none)
+  kMaxit = 14,
   kScalarHeader = 16,  // partials of the last axpy follow the header
 };
 
@@ -45,6 +54,7 @@ struct psg_ctx {
   psg::DevBuf<double> cval, rval;
   psg::DevBuf<float> cval32, rval32;
   int g_csc = 8, g_csr = 32;  // lanes per column / per row
+  int pcg_batch = 4;          // PCG iterations launched between host synchronisations
   int nsplit = 1;             // column blocks of the CSR copy (rptr holds nsplit x (m+1))
   // Newton matrix
   psg::DevBuf<double> theta, w;

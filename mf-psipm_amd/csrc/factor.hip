@@ -102,6 +102,7 @@ struct FactorState {
   DevBuf<double> sf_part64;
   DevBuf<float> sf_part32;
   int solve_mode = 1;  // 1 single-launch, 0 level-scheduled
+  int sf_clean = 0;    // NR of the last completed single-launch solve (its state is cleared); 0: unknown
   int sf_grid = 1024, sf_sleep = 16, sf_batch = 1;  // single-launch launch shape  // 1 if the level holds exactly one supernode
 };
 
@@ -407,6 +408,36 @@ __global__ void k_perm_out(int m, const int* __restrict__ perm, const double* __
     const double d = dsc[i];
 #pragma unroll
     for (int q = 0; q < NR; ++q) z[(int64_t)i * NR + q] = d * (double)x[k * NR + q];
+  }
+}
+
+// The last kernel of a single-launch solve: z = D P' x, and the state the
+// next solve needs cleared -- dependency counters, the backward column
+// sums, the dense tail's accumulation targets -- is zeroed here instead of
+// by separate launches.  Every thread clears only entries no other thread
+// of this launch reads (x entries after it has read them).
+template <class T, int NR = 1>
+__global__ void k_perm_out_reset(int m, const int* __restrict__ perm, const double* __restrict__ dsc,
+                                 T* __restrict__ x, double* __restrict__ z, int* __restrict__ flags, int nflags,
+                                 int err_index, T* __restrict__ acc, T* __restrict__ y, int tail_lo,
+                                 int tail_hi) {
+  const int64_t nmax = max((int64_t)m, (int64_t)nflags);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nmax; k += (int64_t)gridDim.x * blockDim.x) {
+    if (k < m) {
+      const int i = perm[k];
+      const double d = dsc[i];
+      const bool tail = k >= tail_lo && k < tail_hi;
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        z[(int64_t)i * NR + q] = d * (double)x[k * NR + q];
+        acc[k * NR + q] = T(0);
+        if (tail) {
+          x[k * NR + q] = T(0);
+          y[k * NR + q] = T(0);
+        }
+      }
+    }
+    if (k < nflags && k != err_index) flags[k] = 0;  // a timed-out wait stays reported
   }
 }
 
@@ -1306,6 +1337,75 @@ static void zero_async(T* p, int64_t n, hipStream_t st) {
   HIP_LAUNCH_CHECK();
 }
 
+// The same product with 16-byte loads (2 doubles or 4 floats per lane and
+// instruction) and two columns per wave in flight; needs r to be a multiple
+// of the vector width so every column starts 16-byte aligned.
+template <class T> struct Vec16;
+template <> struct Vec16<double> { typedef double v __attribute__((ext_vector_type(2))); static constexpr int n = 2; };
+template <> struct Vec16<float> { typedef float v __attribute__((ext_vector_type(4))); static constexpr int n = 4; };
+
+template <class T, int NR = 1>
+__global__ void __launch_bounds__(kBlock)
+k_tail_upper_vec(int r, const int* __restrict__ tiles, const T* __restrict__ X, const T* __restrict__ y,
+                 T* __restrict__ x) {
+  typedef typename Vec16<T>::v VT;
+  constexpr int V = Vec16<T>::n;
+  __shared__ T ys[NR][kUR];
+  const int J = tiles[2 * blockIdx.x], I = tiles[2 * blockIdx.x + 1];
+  const int i0 = I * kUR;
+  for (int q = threadIdx.x; q < kUR; q += blockDim.x) {
+#pragma unroll
+    for (int e = 0; e < NR; ++e) ys[e][q] = (i0 + q < r) ? y[(int64_t)(i0 + q) * NR + e] : T(0);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  // wave wv takes column pairs (2 wv + 8 k, 2 wv + 8 k + 1)
+  for (int cc = 2 * wv; cc < kUC; cc += 2 * (kBlock / kWave)) {
+    double acc[2][NR];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int e = 0; e < NR; ++e) acc[h][e] = 0.0;
+    int jj[2], ib[2];
+    bool live[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      jj[h] = J * kUC + cc + h;
+      ib[h] = max(i0, jj[h]);
+      live[h] = jj[h] < r && ib[h] < min(i0 + kUR, r);
+    }
+    if (!live[0] && !live[1]) continue;  // uniform per wave
+    const int iend = min(i0 + kUR, r);
+    // vector-aligned start: rows below ib are masked
+    const int start = (min(live[0] ? ib[0] : ib[1], live[1] ? ib[1] : ib[0]) / V) * V;
+    for (int i = start + lane * V; i < iend; i += kWave * V) {
+      VT v[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (live[h]) v[h] = *reinterpret_cast<const VT*>(X + (int64_t)jj[h] * r + i);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (!live[h]) continue;
+#pragma unroll
+        for (int t = 0; t < V; ++t) {
+          const int row = i + t;
+          const double xv = (row >= ib[h] && row < iend) ? (double)v[h][t] : 0.0;
+#pragma unroll
+          for (int e = 0; e < NR; ++e) acc[h][e] += xv * (double)ys[e][min(row - i0, kUR - 1)];
+        }
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int e = 0; e < NR; ++e) {
+        const double sm = wave_sum(acc[h][e]);
+        if (live[h] && lane == 0)
+          __hip_atomic_fetch_add(x + (int64_t)jj[h] * NR + e, (T)sm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+  }
+}
+
 __global__ void k_fill_int(int* __restrict__ p, int n, int v) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
@@ -1488,15 +1588,19 @@ static void dense_factor_inverse(FactorState* fs, hipStream_t st, int r, int F, 
 }
 
 template <class T, int NR = 1>
-static void tail_solve(psg_ctx* c, const T* X, T* vb, T* vy, T* vx) {
+static void tail_solve(psg_ctx* c, const T* X, T* vb, T* vy, T* vx, bool clean) {
   FactorState* fs = c->fs;
   hipStream_t st = c->stream;
   const int r = fs->tail_r, F = fs->tail_F;
-  zero_async(vy + (size_t)F * NR, (int64_t)r * NR, st);
+  if (!clean) zero_async(vy + (size_t)F * NR, (int64_t)r * NR, st);
   hipLaunchKernelGGL((k_tail_lower<T, NR>), dim3(fs->tail_nlower), dim3(kBlock), 0, st, r, fs->tail_lower_tiles.p, X,
                      vb + (size_t)F * NR, vy + (size_t)F * NR);
   HIP_LAUNCH_CHECK();
-  zero_async(vx + (size_t)F * NR, (int64_t)r * NR, st);
+  if (!clean) zero_async(vx + (size_t)F * NR, (int64_t)r * NR, st);
+  if (r % Vec16<T>::n == 0 && reinterpret_cast<uintptr_t>(X) % 16 == 0)
+    hipLaunchKernelGGL((k_tail_upper_vec<T, NR>), dim3(fs->tail_nupper), dim3(kBlock), 0, st, r, fs->tail_upper_tiles.p, X,
+                       vy + (size_t)F * NR, vx + (size_t)F * NR);
+  else
   hipLaunchKernelGGL((k_tail_upper<T, NR>), dim3(fs->tail_nupper), dim3(kBlock), 0, st, r, fs->tail_upper_tiles.p, X,
                      vy + (size_t)F * NR, vx + (size_t)F * NR);
   HIP_LAUNCH_CHECK();
@@ -1923,7 +2027,10 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   hipStream_t s = c->stream;
   const int m = c->m;
   const int ns = fs->nsuper;
-  zero_async(fs->flags.p, (int64_t)fs->flags.n, s);
+  // counters, column sums and tail targets are left cleared by the previous
+  // solve's last kernel; cleared here only after a new factor or a failure
+  const bool clean = fs->sf_clean == NR;
+  if (!clean) zero_async(fs->flags.p, (int64_t)fs->flags.n, s);
   hipLaunchKernelGGL((k_perm_in<T, NR>), dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, fs->perm.p, c->dsc.p,
                      r, vb);
   HIP_LAUNCH_CHECK();
@@ -1957,7 +2064,7 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   a.sleep_max = fs->sf_sleep;
   a.batch = fs->sf_batch;
   a.trace = fs->trace.p;
-  zero_async(a.acc, (int64_t)m * NR, s);
+  if (!clean) zero_async(a.acc, (int64_t)m * NR, s);
   const int grid = std::max(1, std::min(fs->sf_units, fs->sf_grid));
   if (fs->n_iso) {
     hipLaunchKernelGGL((k_iso_solve<T, NR>), dim3(grid_for(fs->n_iso, kBlock, 4096)), dim3(kBlock), 0, s, fs->n_iso,
@@ -1979,9 +2086,9 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   }
   if (fs->tail_nchunk) {
     if constexpr (sizeof(T) == 8)
-      tail_solve<double, NR>(c, fs->tail_L64.p, vb, vy, vx);
+      tail_solve<double, NR>(c, fs->tail_L64.p, vb, vy, vx, clean);
     else
-      tail_solve<float, NR>(c, fs->tail_L32.p, vb, vy, vx);
+      tail_solve<float, NR>(c, fs->tail_L32.p, vb, vy, vx, clean);
   }
   if (fs->sf_units) {
     hipLaunchKernelGGL((k_bwd_sf<T, NR>), dim3(grid), dim3(kBlock), 0, s, a);
@@ -1994,9 +2101,14 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
                        fs->doff.p, panels, dinv, vy, vx);
     HIP_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL((k_perm_out<T, NR>), dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, fs->perm.p,
-                     c->dsc.p, vx, z);
-  HIP_LAUNCH_CHECK();
+  {
+    const int tail_lo = fs->tail_nchunk ? fs->tail_F : 0, tail_hi = fs->tail_nchunk ? fs->tail_F + fs->tail_r : 0;
+    const int nflags = (int)fs->flags.n;
+    hipLaunchKernelGGL((k_perm_out_reset<T, NR>), dim3(grid_for(std::max(m, nflags), kBlock, 4096)), dim3(kBlock), 0, s,
+                       m, fs->perm.p, c->dsc.p, vx, z, fs->flags.p, nflags, 4 * ns + 4, a.acc, vy, tail_lo, tail_hi);
+    HIP_LAUNCH_CHECK();
+    fs->sf_clean = NR;
+  }
 }
 
 int* factor_knob(psg_ctx* c, int knob) {
@@ -2016,6 +2128,7 @@ int factor_solve_error(psg_ctx* c) {
   int e = 0;
   HIP_CHECK(hipMemcpyAsync(&e, fs->flags.p + 4 * fs->nsuper + 4, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (e) fs->sf_clean = 0;  // the next solve clears everything, the error flag included
   return e;
 }
 
@@ -2040,6 +2153,7 @@ int factor_numeric(psg_ctx* c, const Plan& P, double eta, double* nnz_factor) {
   if (P.m != c->m) throw Error(PSG_ERR_ARGUMENT, "plan dimension does not match the context");
   if (c->fs->serial != P.serial || c->fs->precision != c->precision) upload_plan(c, P);
   c->fs->ready = false;
+  c->fs->sf_clean = 0;
   HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   int status;
   if (c->precision == PSG_MIXED)

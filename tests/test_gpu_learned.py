@@ -129,14 +129,15 @@ def test_learned_selection_against_the_rule(gpu):
     m, n = 600, 20000
     _say(f"learned vs rule, set cover {m}x{n}")
     train = [setcover_problem(m, n, s) for s in (1000, 1001)]
-    X, y = collect_trajectories([(lp, tol) for _, lp, tol in train], per_iter=8000)
+    opts = dict(full_preconditioner_row_threshold=0)  # the rule's core, not the exact factor
+    X, y = collect_trajectories([(lp, tol) for _, lp, tol in train], per_iter=8000, **opts)
     assert X.shape[0] > 1000 and 0.0 < y.mean() < 0.5
     sel = LearnedColumnSelector(seed=0)
     losses = sel.fit(X, y, steps=600, batch=2048, lr=2e-3)
     assert losses[-50:].mean() < losses[:20].mean()
     raw, lp, tol = setcover_problem(m, n, 0)
-    sol_r, rule = solve_with(lp, tol)
-    sol_l, learned = solve_with(lp, tol, selector=sel)
+    sol_r, rule = solve_with(lp, tol, **opts)
+    sol_l, learned = solve_with(lp, tol, selector=sel, **opts)
     _say(f"rule: pcg {rule['pcg']} factorizations {rule['factorizations']} | learned: pcg {learned['pcg']} "
          f"factorizations {learned['factorizations']} learned refreshes {learned['learned_refreshes']} "
          f"fallbacks {learned['learned_fallbacks']}")

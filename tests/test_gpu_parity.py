@@ -444,3 +444,39 @@ def test_network_grid_matches_highs(gpu):
     # the grounded rows are redundant: the full balance holds too
     assert np.abs(full.b - full.A @ sol.x).max() <= 1e-8 * (1.0 + np.abs(full.b).max())
     solver.close()
+
+
+@pytest.mark.parametrize("replacement,maxit", [(100, 200), (7, 200), (0, 25)])
+def test_pcg_launch_batch_does_not_change_the_iterates(gpu, replacement, maxit):
+    """The device-judged PCG loop holds for the host at every explicit check,
+    replacement and the iteration limit, so launching 1, 4 or 16 iterations
+    between synchronisations gives the same outcome (up to the forward
+    solve's non-fixed atomic order)."""
+    _say(f"pcg launch batch replacement={replacement} maxit={maxit}")
+    from mf_psipm_amd import autotune as AT
+    _, lp = _lp(1600, 4800, 5, 91)
+    obj = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, cg_tol=1e-10, cg_tol_max=1e-10,
+                                                cg_maxit=maxit, structural_backbone=False, core_factor=0.4)
+    obj.residual_replacement = replacement
+    obj.factorize(_theta(4800, 92, 2.5), 1e-6)
+    rng = np.random.default_rng(93)
+    r1, r2 = rng.standard_normal(4800), rng.standard_normal(1600)
+    out = []
+    for batch in (1, 4, 16):
+        AT.set_(obj, 5, batch)
+        obj._call("psg_solve_begin", obj._ctx, r1, r2)
+        res = obj._pcg(1e-10)
+        dx, dy = np.empty(4800), np.empty(1600)
+        obj._call("psg_solve_finish", obj._ctx, dx, dy)
+        out.append((res, dy))
+        _say(f"batch {batch}: info {res.info} iterations {res.iterations} checks {res.residual_checks} "
+             f"replacements {res.replacements} relres {res.relative_residual:.3e}")
+    r0, y0 = out[0]
+    for res, dy in out[1:]:
+        # the forward solve's atomic accumulation order is not fixed, so a
+        # tolerance crossing can move by an iteration either way
+        assert res.info == r0.info
+        assert abs(res.iterations - r0.iterations) <= 2
+        assert abs(res.residual_checks - r0.residual_checks) <= 1
+        assert np.linalg.norm(dy - y0) <= 1e-7 * np.linalg.norm(y0)
+    obj.close()
