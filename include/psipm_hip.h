@@ -218,6 +218,9 @@ int psg_price(psg_ctx* ctx, const double* y, double* violation);
  * a device-side judge holds the loop wherever the host must act, so the
  * iterates do not depend on it */
 #define PSG_KNOB_PCG_BATCH 5
+/* 1 (default): the PCG iteration is captured once as a HIP graph and
+ * replayed; 0: plain launches */
+#define PSG_KNOB_PCG_GRAPH 6
 int psg_knob_get(psg_ctx* ctx, int32_t knob, int32_t* value);
 int psg_knob_set(psg_ctx* ctx, int32_t knob, int32_t value);
 

@@ -700,6 +700,8 @@ int psg_destroy(psg_ctx* c) {
   if (!c) return PSG_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   factor_destroy(c);
+  if (c->pcg_exec) (void)hipGraphExecDestroy(c->pcg_exec);
+  if (c->pcg_graph) (void)hipGraphDestroy(c->pcg_graph);
   delete c->b2;
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -721,6 +723,7 @@ int psg_set_weights(psg_ctx* c, const double* theta, double delta) {
       c->theta.grow(c->n);
     }
     c->nhist = std::min(3, c->nhist + 1);
+    c->gen++;  // delta is a launch argument of the captured iteration
     c->theta.upload(theta, c->n, c->stream);
     hipLaunchKernelGGL(k_recip, dim3(grid_for(c->n, kBlock, 4096)), dim3(kBlock), 0, c->stream, c->n,
                        c->theta.p, c->w.p);
@@ -874,12 +877,40 @@ int psg_pcg(psg_ctx* c, double tol, int32_t maxit, int32_t replacement, double g
       HIP_LAUNCH_CHECK();
     };
     const int batch = std::max(1, c->pcg_batch);
+    // k iterations: graph replays of the captured iteration, or plain launches
+    auto run = [&](int k) {
+      if (!c->pcg_use_graph) {
+        for (int q = 0; q < k; ++q) iteration();
+        return;
+      }
+      const void* bufs[3] = {c->rzs.p, c->alphas.p, c->betas.p};
+      if (!c->pcg_exec || c->pcg_gen != c->gen || !std::equal(bufs, bufs + 3, c->pcg_bufs)) {
+        if (c->pcg_exec) (void)hipGraphExecDestroy(c->pcg_exec);
+        if (c->pcg_graph) (void)hipGraphDestroy(c->pcg_graph);
+        c->pcg_exec = nullptr;
+        c->pcg_graph = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          iteration();
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(s, &c->pcg_graph));
+        HIP_CHECK(hipGraphInstantiate(&c->pcg_exec, c->pcg_graph, nullptr, nullptr, 0));
+        c->pcg_gen = c->gen;
+        std::copy(bufs, bufs + 3, c->pcg_bufs);
+      }
+      for (int q = 0; q < k; ++q) HIP_CHECK(hipGraphLaunch(c->pcg_exec, s));
+    };
     while (info > 0 && it < maxit) {
       // iterations until the next replacement boundary or the limit: the
       // judge holds there anyway, so a batch never runs past it
       int k = std::min(batch, maxit - it);
       if (replacement > 0) k = std::min(k, replacement - it % replacement);
-      for (int q = 0; q < k; ++q) iteration();
+      run(k);
       HIP_CHECK(hipMemcpyAsync(c->hscal, c->scal.p, kScalarHeader * sizeof(double), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
       it = (int)c->hscal[kIt];
@@ -2021,6 +2052,7 @@ extern "C" int psg_lowrank_set(psg_ctx* c, int32_t q, const int32_t* cols, const
     if (!c || q < 0 || q > kMaxLowRank || (q && (!cols || !U)))
       throw Error(PSG_ERR_ARGUMENT, "psg_lowrank_set: invalid arguments (at most 8 columns)");
     hipStream_t s = c->stream;
+    c->gen++;  // the correction is part of the captured PCG iteration
     if (q == 0) {
       c->lr_q = 0;
       c->lr_active = false;
@@ -2058,6 +2090,7 @@ int psg_knob_get(psg_ctx* c, int32_t knob, int32_t* value) {
     if (!c || !value) throw Error(PSG_ERR_ARGUMENT, "psg_knob_get: null argument");
     if (knob == PSG_KNOB_G_CSC) { *value = c->g_csc; return PSG_OK; }
     if (knob == PSG_KNOB_PCG_BATCH) { *value = c->pcg_batch; return PSG_OK; }
+    if (knob == PSG_KNOB_PCG_GRAPH) { *value = c->pcg_use_graph; return PSG_OK; }
     if (knob == PSG_KNOB_G_CSR) { *value = c->g_csr; return PSG_OK; }
     int* p = factor_knob(c, knob);
     if (!p) throw Error(knob >= 2 && knob <= 4 ? PSG_ERR_STATE : PSG_ERR_ARGUMENT, "psg_knob_get: unknown knob or no factor");
@@ -2069,7 +2102,13 @@ int psg_knob_get(psg_ctx* c, int32_t knob, int32_t* value) {
 int psg_knob_set(psg_ctx* c, int32_t knob, int32_t value) {
   return guarded([&]() -> int {
     if (!c) throw Error(PSG_ERR_ARGUMENT, "psg_knob_set: null context");
+    c->gen++;  // knobs are launch arguments of the captured PCG iteration
     auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+    if (knob == PSG_KNOB_PCG_GRAPH) {
+      if (value != 0 && value != 1) throw Error(PSG_ERR_ARGUMENT, "psg_knob_set: PCG graph is 0 or 1");
+      c->pcg_use_graph = value;
+      return PSG_OK;
+    }
     if (knob == PSG_KNOB_PCG_BATCH) {
       if (value < 1 || value > 64) throw Error(PSG_ERR_ARGUMENT, "psg_knob_set: PCG batch must lie in [1, 64]");
       c->pcg_batch = value;

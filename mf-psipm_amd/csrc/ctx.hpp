@@ -55,6 +55,14 @@ struct psg_ctx {
   psg::DevBuf<float> cval32, rval32;
   int g_csc = 8, g_csr = 32;  // lanes per column / per row
   int pcg_batch = 4;          // PCG iterations launched between host synchronisations
+  // the PCG iteration captured as a HIP graph; `gen` changes whenever a
+  // captured launch argument may have (weights, factor, knobs, low-rank
+  // state, a solve error), and the graph is captured again
+  int pcg_use_graph = 1;
+  uint64_t gen = 1, pcg_gen = 0;
+  hipGraph_t pcg_graph = nullptr;
+  hipGraphExec_t pcg_exec = nullptr;
+  const void* pcg_bufs[3] = {nullptr, nullptr, nullptr};
   int nsplit = 1;             // column blocks of the CSR copy (rptr holds nsplit x (m+1))
   // Newton matrix
   psg::DevBuf<double> theta, w;

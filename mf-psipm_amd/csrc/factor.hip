@@ -90,6 +90,7 @@ struct FactorState {
   int tail_r = 0, tail_F = 0, tail_s0 = 0, tail_nchunk = 0;
   DevBuf<int> tail_chunk_of, tail_lower_tiles, tail_upper_tiles;
   int tail_nlower = 0, tail_nupper = 0;
+  int tail_tc = 512, tail_ur = 1024;  // lower-tile columns, upper-tile rows
   DevBuf<double> tail_L64, tail_S64;
   DevBuf<float> tail_L32, tail_S32;
   struct TailStep { int kind, j, w; MGemm g; };  // kind: 0 potrf, 1 trsm, 2 update
@@ -1241,19 +1242,19 @@ k_dense_trsm(T* __restrict__ L, int r, int j, int w) {
 constexpr int kTR = 256, kTC = 512;
 template <class T, int NR = 1>
 __global__ void __launch_bounds__(kBlock)
-k_tail_lower(int r, const int* __restrict__ tiles, const T* __restrict__ X, const T* __restrict__ b,
+k_tail_lower(int r, int tc, const int* __restrict__ tiles, const T* __restrict__ X, const T* __restrict__ b,
              T* __restrict__ y) {
   __shared__ T bs[NR][kTC];
   const int I = tiles[2 * blockIdx.x], J = tiles[2 * blockIdx.x + 1];
-  const int j0 = J * kTC;
-  for (int q = threadIdx.x; q < kTC; q += blockDim.x) {
+  const int j0 = J * tc;
+  for (int q = threadIdx.x; q < tc; q += blockDim.x) {
 #pragma unroll
     for (int e = 0; e < NR; ++e) bs[e][q] = (j0 + q < r) ? b[(int64_t)(j0 + q) * NR + e] : T(0);
   }
   __syncthreads();
   const int i = I * kTR + threadIdx.x;
   if (i >= r) return;
-  const int jend = min(min(j0 + kTC, r), i + 1);
+  const int jend = min(min(j0 + tc, r), i + 1);
   double acc[NR][4];
 #pragma unroll
   for (int e = 0; e < NR; ++e) acc[e][0] = acc[e][1] = acc[e][2] = acc[e][3] = 0.0;
@@ -1288,12 +1289,12 @@ k_tail_lower(int r, const int* __restrict__ tiles, const T* __restrict__ X, cons
 constexpr int kUC = 64, kUR = 1024;
 template <class T, int NR = 1>
 __global__ void __launch_bounds__(kBlock)
-k_tail_upper(int r, const int* __restrict__ tiles, const T* __restrict__ X, const T* __restrict__ y,
+k_tail_upper(int r, int ur, const int* __restrict__ tiles, const T* __restrict__ X, const T* __restrict__ y,
              T* __restrict__ x) {
   __shared__ T ys[NR][kUR];
   const int J = tiles[2 * blockIdx.x], I = tiles[2 * blockIdx.x + 1];
-  const int i0 = I * kUR;
-  for (int q = threadIdx.x; q < kUR; q += blockDim.x) {
+  const int i0 = I * ur;
+  for (int q = threadIdx.x; q < ur; q += blockDim.x) {
 #pragma unroll
     for (int e = 0; e < NR; ++e) ys[e][q] = (i0 + q < r) ? y[(int64_t)(i0 + q) * NR + e] : T(0);
   }
@@ -1302,7 +1303,7 @@ k_tail_upper(int r, const int* __restrict__ tiles, const T* __restrict__ X, cons
   for (int cc = wv; cc < kUC; cc += kBlock / kWave) {
     const int j = J * kUC + cc;
     if (j >= r) break;
-    const int ibeg = max(i0, j), iend = min(i0 + kUR, r);
+    const int ibeg = max(i0, j), iend = min(i0 + ur, r);
     if (ibeg >= iend) continue;
     double acc[NR];
 #pragma unroll
@@ -1346,14 +1347,14 @@ template <> struct Vec16<float> { typedef float v __attribute__((ext_vector_type
 
 template <class T, int NR = 1>
 __global__ void __launch_bounds__(kBlock)
-k_tail_upper_vec(int r, const int* __restrict__ tiles, const T* __restrict__ X, const T* __restrict__ y,
+k_tail_upper_vec(int r, int ur, const int* __restrict__ tiles, const T* __restrict__ X, const T* __restrict__ y,
                  T* __restrict__ x) {
   typedef typename Vec16<T>::v VT;
   constexpr int V = Vec16<T>::n;
   __shared__ T ys[NR][kUR];
   const int J = tiles[2 * blockIdx.x], I = tiles[2 * blockIdx.x + 1];
-  const int i0 = I * kUR;
-  for (int q = threadIdx.x; q < kUR; q += blockDim.x) {
+  const int i0 = I * ur;
+  for (int q = threadIdx.x; q < ur; q += blockDim.x) {
 #pragma unroll
     for (int e = 0; e < NR; ++e) ys[e][q] = (i0 + q < r) ? y[(int64_t)(i0 + q) * NR + e] : T(0);
   }
@@ -1372,10 +1373,10 @@ k_tail_upper_vec(int r, const int* __restrict__ tiles, const T* __restrict__ X, 
     for (int h = 0; h < 2; ++h) {
       jj[h] = J * kUC + cc + h;
       ib[h] = max(i0, jj[h]);
-      live[h] = jj[h] < r && ib[h] < min(i0 + kUR, r);
+      live[h] = jj[h] < r && ib[h] < min(i0 + ur, r);
     }
     if (!live[0] && !live[1]) continue;  // uniform per wave
-    const int iend = min(i0 + kUR, r);
+    const int iend = min(i0 + ur, r);
     // vector-aligned start: rows below ib are masked
     const int start = (min(live[0] ? ib[0] : ib[1], live[1] ? ib[1] : ib[0]) / V) * V;
     for (int i = start + lane * V; i < iend; i += kWave * V) {
@@ -1391,7 +1392,7 @@ k_tail_upper_vec(int r, const int* __restrict__ tiles, const T* __restrict__ X, 
           const int row = i + t;
           const double xv = (row >= ib[h] && row < iend) ? (double)v[h][t] : 0.0;
 #pragma unroll
-          for (int e = 0; e < NR; ++e) acc[h][e] += xv * (double)ys[e][min(row - i0, kUR - 1)];
+          for (int e = 0; e < NR; ++e) acc[h][e] += xv * (double)ys[e][min(row - i0, ur - 1)];
         }
       }
     }
@@ -1427,11 +1428,23 @@ static void setup_tail(psg_ctx* c, const Plan& P) {
     for (int col = P.sfirst[sn]; col < P.sfirst[sn + 1]; ++col) chunk_of[col - F] = sn;
   }
   fs->tail_chunk_of.upload(chunk_of, st);
+  // tile widths shrink with the tail so that a small tail still spreads over
+  // ~1000 workgroups: columns per lower tile tc (rows fixed at kTR), rows per
+  // upper tile ur (columns fixed at kUC); powers of two in [32, kTC] / [64, kUR]
+  auto pick = [](double want, int lo, int hi) {
+    int v = lo;
+    while (v < hi && v < want) v <<= 1;
+    return v;
+  };
+  const double rr = (double)r * (double)r;
+  fs->tail_tc = pick(rr / (2.0 * kTR * 1024.0), 32, kTC);
+  fs->tail_ur = pick(rr / (2.0 * kUC * 1024.0), 64, kUR);
+  const int tc = fs->tail_tc, ur = fs->tail_ur;
   std::vector<int> lt, ut;
   for (int I = 0; I * kTR < r; ++I)
-    for (int J = 0; J * kTC < std::min(r, (I + 1) * kTR); ++J) { lt.push_back(I); lt.push_back(J); }
+    for (int J = 0; J * tc < std::min(r, (I + 1) * kTR); ++J) { lt.push_back(I); lt.push_back(J); }
   for (int J = 0; J * kUC < r; ++J)
-    for (int I = (J * kUC) / kUR; I * kUR < r; ++I) { ut.push_back(J); ut.push_back(I); }
+    for (int I = (J * kUC) / ur; I * ur < r; ++I) { ut.push_back(J); ut.push_back(I); }
   fs->tail_nlower = (int)lt.size() / 2;
   fs->tail_nupper = (int)ut.size() / 2;
   fs->tail_lower_tiles.upload(lt, st);
@@ -1593,15 +1606,18 @@ static void tail_solve(psg_ctx* c, const T* X, T* vb, T* vy, T* vx, bool clean) 
   hipStream_t st = c->stream;
   const int r = fs->tail_r, F = fs->tail_F;
   if (!clean) zero_async(vy + (size_t)F * NR, (int64_t)r * NR, st);
-  hipLaunchKernelGGL((k_tail_lower<T, NR>), dim3(fs->tail_nlower), dim3(kBlock), 0, st, r, fs->tail_lower_tiles.p, X,
+  hipLaunchKernelGGL((k_tail_lower<T, NR>), dim3(fs->tail_nlower), dim3(kBlock), 0, st, r, fs->tail_tc,
+                     fs->tail_lower_tiles.p, X,
                      vb + (size_t)F * NR, vy + (size_t)F * NR);
   HIP_LAUNCH_CHECK();
   if (!clean) zero_async(vx + (size_t)F * NR, (int64_t)r * NR, st);
   if (r % Vec16<T>::n == 0 && reinterpret_cast<uintptr_t>(X) % 16 == 0)
-    hipLaunchKernelGGL((k_tail_upper_vec<T, NR>), dim3(fs->tail_nupper), dim3(kBlock), 0, st, r, fs->tail_upper_tiles.p, X,
+    hipLaunchKernelGGL((k_tail_upper_vec<T, NR>), dim3(fs->tail_nupper), dim3(kBlock), 0, st, r, fs->tail_ur,
+                       fs->tail_upper_tiles.p, X,
                        vy + (size_t)F * NR, vx + (size_t)F * NR);
   else
-  hipLaunchKernelGGL((k_tail_upper<T, NR>), dim3(fs->tail_nupper), dim3(kBlock), 0, st, r, fs->tail_upper_tiles.p, X,
+  hipLaunchKernelGGL((k_tail_upper<T, NR>), dim3(fs->tail_nupper), dim3(kBlock), 0, st, r, fs->tail_ur,
+                     fs->tail_upper_tiles.p, X,
                      vy + (size_t)F * NR, vx + (size_t)F * NR);
   HIP_LAUNCH_CHECK();
   // tail supernodes count as solved for the backward dependency waits
@@ -1631,16 +1647,22 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
   // dense tail: a postorder suffix [s0, q] of the subtree of a root q is
   // closed under "parent of", so its columns [F, E) form the trailing
   // diagonal block of L with nothing else below it.  The longest suffix
-  // whose block holds at least a fraction PSG_TAIL_DENSITY of nonzeros is
-  // inverted explicitly when it has PSG_TAIL_MIN columns or more.
+  // whose block holds at least a fraction PSG_TAIL_DENSITY of nonzeros, or
+  // whose dense triangle has at most PSG_TAIL_SMALL entries, is inverted
+  // explicitly when it has PSG_TAIL_MIN columns or more.
   if (fs->solve_mode == 1) {
     auto knobd = [](const char* name, double dflt) {
       const char* v = getenv(name);
       return v ? atof(v) : dflt;
     };
-    const int tail_min = (int)knobd("PSG_TAIL_MIN", 2048);
+    // a tail of a few hundred columns already replaces a long dependent
+    // chain of the single-launch solve by two bandwidth-bound products
+    const int tail_min = (int)knobd("PSG_TAIL_MIN", 128);
     const int tail_max = (int)knobd("PSG_TAIL_MAX", 24576);
     const double rho = knobd("PSG_TAIL_DENSITY", 0.8);
+    // below this dense size (entries) a suffix is inverted whatever its
+    // density: streaming a few MB beats the latency of the dependency chain
+    const double small = knobd("PSG_TAIL_SMALL", 4.0e6);
     std::vector<int> sub(P.nsuper, 1);  // subtree sizes (postorder: children first)
     for (int sn = 0; sn < P.nsuper; ++sn)
       if (P.sparent[sn] >= 0) sub[P.sparent[sn]] += sub[sn];
@@ -1655,9 +1677,11 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
         nz += w * nr - w * (w - 1) / 2;
         const double r = E - P.sfirst[sn];
         if (r > tail_max) break;
-        const double dens = nz / (r * (r + 1) / 2);
-        if (dens >= rho && r > best_r) { best_r = (int)r; best_q = q; best_s0 = sn; }
-        if (dens < 0.5 * rho) break;
+        const double dense_entries = r * (r + 1) / 2;
+        const double dens = nz / dense_entries;
+        const bool take = dens >= rho || dense_entries <= small;
+        if (take && r > best_r) { best_r = (int)r; best_q = q; best_s0 = sn; }
+        if (dens < 0.5 * rho && dense_entries > small) break;
       }
     }
     fs->tail_r = 0;
@@ -2128,7 +2152,10 @@ int factor_solve_error(psg_ctx* c) {
   int e = 0;
   HIP_CHECK(hipMemcpyAsync(&e, fs->flags.p + 4 * fs->nsuper + 4, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
-  if (e) fs->sf_clean = 0;  // the next solve clears everything, the error flag included
+  if (e) {
+    fs->sf_clean = 0;  // the next solve clears everything, the error flag included
+    c->gen++;
+  }
   return e;
 }
 
@@ -2154,6 +2181,7 @@ int factor_numeric(psg_ctx* c, const Plan& P, double eta, double* nnz_factor) {
   if (c->fs->serial != P.serial || c->fs->precision != c->precision) upload_plan(c, P);
   c->fs->ready = false;
   c->fs->sf_clean = 0;
+  c->gen++;  // the factor's layout and buffers are captured PCG launch arguments
   HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   int status;
   if (c->precision == PSG_MIXED)

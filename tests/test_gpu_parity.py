@@ -450,8 +450,9 @@ def test_network_grid_matches_highs(gpu):
 def test_pcg_launch_batch_does_not_change_the_iterates(gpu, replacement, maxit):
     """The device-judged PCG loop holds for the host at every explicit check,
     replacement and the iteration limit, so launching 1, 4 or 16 iterations
-    between synchronisations gives the same outcome (up to the forward
-    solve's non-fixed atomic order)."""
+    between synchronisations, as plain launches or as replays of the
+    captured iteration, gives the same outcome (up to the forward solve's
+    non-fixed atomic order)."""
     _say(f"pcg launch batch replacement={replacement} maxit={maxit}")
     from mf_psipm_amd import autotune as AT
     _, lp = _lp(1600, 4800, 5, 91)
@@ -462,14 +463,15 @@ def test_pcg_launch_batch_does_not_change_the_iterates(gpu, replacement, maxit):
     rng = np.random.default_rng(93)
     r1, r2 = rng.standard_normal(4800), rng.standard_normal(1600)
     out = []
-    for batch in (1, 4, 16):
+    for batch, graph in ((1, 0), (4, 1), (16, 1), (4, 0), (1, 1)):
         AT.set_(obj, 5, batch)
+        AT.set_(obj, 6, graph)
         obj._call("psg_solve_begin", obj._ctx, r1, r2)
         res = obj._pcg(1e-10)
         dx, dy = np.empty(4800), np.empty(1600)
         obj._call("psg_solve_finish", obj._ctx, dx, dy)
         out.append((res, dy))
-        _say(f"batch {batch}: info {res.info} iterations {res.iterations} checks {res.residual_checks} "
+        _say(f"batch {batch} graph {graph}: info {res.info} iterations {res.iterations} checks {res.residual_checks} "
              f"replacements {res.replacements} relres {res.relative_residual:.3e}")
     r0, y0 = out[0]
     for res, dy in out[1:]:
