@@ -1,0 +1,82 @@
+"""Write the golden fixtures of the parity tests (tests/golden/*.npz).
+
+The reference ships no test vectors (its hot path needs CHOLMOD, absent
+here), so the fixtures are made from what fixes the answers independently:
+scipy's HiGHS for LP optima, LAPACK for the preconditioner solve, scipy
+sparse products for the operator, and the float64 MLP of oracle/mlp_cpu.py
+(itself pinned by finite differences) for the selector network.
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+import scipy.linalg as la
+import scipy.sparse as sp
+from scipy.optimize import linprog
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import mf_psipm_amd as P  # noqa: E402
+from oracle import mlp_cpu as M  # noqa: E402
+
+
+def lp_fixture():
+    """LP optima of the 2000 x 6000 random workload (seeds 0, 1) and of a
+    set-covering LP, by HiGHS dual simplex on the raw problem."""
+    out = {}
+    for seed in (0, 1):
+        raw, x0, y0, z0 = P.generators.random_lp(2000, 6000, 5, seed)
+        ref = linprog(raw.c, A_eq=raw.A, b_eq=raw.rl, bounds=(0, None), method="highs-ds")
+        out[f"random_2000x6000_s{seed}_obj"] = ref.fun
+        out[f"random_2000x6000_s{seed}_sandwich"] = np.array([raw.rl @ y0, raw.c @ x0])
+    raw = P.generators.set_covering(300, 6000, 7)
+    ref = linprog(raw.c, A_ub=-raw.A, b_ub=-raw.rl, bounds=(0, None), method="highs-ds")
+    out["setcover_300x6000_s7_obj"] = ref.fun
+    return out
+
+
+def operator_fixture():
+    """H v and P^-1 r (LAPACK) of a 400 x 1200 LP for fixed weights and core."""
+    raw, *_ = P.generators.random_lp(400, 1200, 5, 11)
+    lp = P.lp.to_standard_form(raw)
+    P.lp.scale_problem(lp)
+    A = lp.A
+    rng = np.random.default_rng(12)
+    theta = 10.0 ** rng.uniform(-2, 2, A.shape[1])
+    w = 1.0 / theta
+    delta = 1e-3
+    v = rng.standard_normal(A.shape[0])
+    Hv = A @ (w * (A.T @ v)) + delta * v
+    cand = np.flatnonzero(np.diff(A.indptr) >= 2)
+    core = np.sort(rng.choice(cand, 250, replace=False))
+    Ac = A[:, core]
+    G = (Ac @ sp.diags(w[core]) @ Ac.T).toarray()
+    d_full = A.multiply(A) @ w
+    Mm = G + np.diag(np.maximum(d_full - np.diag(G), 0.0) + delta)
+    dsc = 1.0 / np.sqrt(np.diag(Mm))
+    Ms = dsc[:, None] * Mm * dsc[None, :]
+    z = dsc * la.cho_solve(la.cho_factor(Ms, lower=True), dsc * v)
+    return dict(op_theta=theta, op_delta=np.array(delta), op_v=v, op_Hv=Hv, op_core=core.astype(np.int32), op_Pinv_v=z)
+
+
+def mlp_fixture():
+    """Logits, loss and gradient of the selector network for fixed inputs."""
+    rng = np.random.default_rng(21)
+    params = M.init_params(21)
+    params[M.OFF_B1:M.OFF_W2] = rng.uniform(-0.1, 0.1, 64)
+    X = rng.standard_normal((257, 8)).astype(np.float32)
+    y = (rng.uniform(size=257) < 0.3).astype(np.float32)
+    z, _ = M.forward(params.astype(np.float64), X.astype(np.float64))
+    L, g = M.loss_and_grad(params.astype(np.float64), X.astype(np.float64), y, 2.0)
+    return dict(mlp_params=params, mlp_X=X, mlp_y=y, mlp_logits=z, mlp_loss=np.array(L), mlp_grad=g)
+
+
+if __name__ == "__main__":
+    np.savez_compressed(os.path.join(HERE, "lp_optima.npz"), **lp_fixture())
+    np.savez_compressed(os.path.join(HERE, "operator_400x1200.npz"), **operator_fixture())
+    np.savez_compressed(os.path.join(HERE, "mlp_selector.npz"), **mlp_fixture())
+    print("written to", HERE)

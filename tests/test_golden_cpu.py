@@ -1,0 +1,66 @@
+"""The oracle against the committed golden fixtures (tests/golden/).
+
+The fixtures come from HiGHS (LP optima), LAPACK (the preconditioner solve),
+scipy products (the operator) and the float64 selector network; the script
+that wrote them is tests/golden/make_golden.py.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import mf_psipm_amd as P
+from oracle import mlp_cpu as M
+from oracle import ssp_cpu as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def _operator_lp():
+    raw, *_ = P.generators.random_lp(400, 1200, 5, 11)
+    lp = P.lp.to_standard_form(raw)
+    P.lp.scale_problem(lp)
+    return lp
+
+
+def test_oracle_operator_and_preconditioner_match_golden():
+    g = _load("operator_400x1200.npz")
+    lp = _operator_lp()
+    cpu = O.SpectralStableColumnPreconditionerCPU(lp.A, protected=~lp.pos, full_preconditioner_row_threshold=0)
+    theta, delta = g["op_theta"], float(g["op_delta"])
+    cpu._weights = 1.0 / theta
+    cpu.delta = delta
+    assert np.allclose(cpu.full_matvec(g["op_v"]), g["op_Hv"], rtol=1e-13, atol=1e-13 * np.abs(g["op_Hv"]).max())
+    cpu.core = g["op_core"].astype(np.int64)
+    cpu._factor_numeric(theta, delta)
+    z = cpu.apply(g["op_v"])
+    assert np.linalg.norm(z - g["op_Pinv_v"]) <= 1e-10 * np.linalg.norm(g["op_Pinv_v"])
+
+
+def test_oracle_lp_optimum_matches_golden():
+    g = _load("lp_optima.npz")
+    raw, *_ = P.generators.random_lp(2000, 6000, 5, 1)
+    lp = P.lp.to_standard_form(raw)
+    P.lp.scale_problem(lp)
+    tol = P.lp.scaled_tolerance(lp, 1e-8)
+    kkt = O.SpectralStableColumnPreconditionerCPU(lp.A, protected=~lp.pos, cg_tol=1e-8, cg_tol_max=1e-8)
+    sol = P.ipm.psipm_solve(lp, kkt, tol=tol, compl="gap", maxit=160)
+    info = P.lp.recover(lp, raw, sol.x, sol.y, sol.z, sol.w)
+    ref = float(g["random_2000x6000_s1_obj"])
+    lo, hi = g["random_2000x6000_s1_sandwich"]
+    assert sol.status == "optimal"
+    assert abs(info["obj"] - ref) <= 1e-8 * abs(ref)
+    assert lo - 1e-9 * abs(lo) <= ref <= hi + 1e-9 * abs(hi)
+
+
+def test_oracle_network_matches_golden():
+    g = _load("mlp_selector.npz")
+    z, _ = M.forward(g["mlp_params"].astype(np.float64), g["mlp_X"].astype(np.float64))
+    assert np.allclose(z, g["mlp_logits"], rtol=1e-12, atol=1e-12)
+    L, grad = M.loss_and_grad(g["mlp_params"].astype(np.float64), g["mlp_X"].astype(np.float64), g["mlp_y"], 2.0)
+    assert abs(L - float(g["mlp_loss"])) <= 1e-12
+    assert np.allclose(grad, g["mlp_grad"], rtol=1e-10, atol=1e-14)
