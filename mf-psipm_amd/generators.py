@@ -125,3 +125,43 @@ def set_covering(m: int, n: int, seed: int, k_lo: int = 2, k_hi: int = 10) -> Ra
         c=c, A=A.tocsc(), rl=np.ones(m), ru=np.full(m, INF), cl=np.zeros(n),
         cu=np.full(n, INF), name=f"setcover_{m}x{n}_s{seed}",
     )
+
+
+def mixed_bounds_lp(m: int, n: int, nnz_per_col: int, seed: int) -> RawLP:
+    """Random LP exercising every bound and row type of the pipeline.
+
+    Columns: 40% ``[0, inf)``, 30% boxed ``[l, u]`` around an interior point,
+    10% free, 10% upper-bounded only, 10% with a negative finite lower bound;
+    rows: 70% equalities, 30% ranges around ``A x0``.  The costs make ``y0``
+    dual feasible (free columns get exactly ``(A'y0)_j``), so the LP is
+    feasible and bounded.
+    """
+    rng = np.random.default_rng(seed)
+    indptr, indices = _distinct_rows(rng, m, n, nnz_per_col)
+    A = sp.csc_matrix((rng.standard_normal(indices.size), indices, indptr), shape=(m, n))
+    kind = rng.choice(5, size=n, p=[0.4, 0.3, 0.1, 0.1, 0.1])
+    x0 = rng.uniform(0.1, 1.0, n)
+    x0[kind == 3] = -x0[kind == 3]
+    cl = np.zeros(n)
+    cu = np.full(n, INF)
+    box = kind == 1
+    cl[box] = x0[box] - rng.uniform(0.1, 1.0, box.sum())
+    cu[box] = x0[box] + rng.uniform(0.1, 1.0, box.sum())
+    cl[kind == 2] = -INF
+    cl[kind == 3] = -INF
+    cu[kind == 3] = 0.0
+    neg = kind == 4
+    cl[neg] = -rng.uniform(0.5, 2.0, neg.sum())
+    y0 = rng.standard_normal(m)
+    z0 = rng.uniform(0.1, 1.0, n)
+    aty = A.T @ y0
+    c = aty + z0
+    c[kind == 1] = aty[kind == 1] + rng.standard_normal(box.sum())
+    c[kind == 2] = aty[kind == 2]
+    c[kind == 3] = aty[kind == 3] - z0[kind == 3]
+    ax = A @ x0
+    rng_rows = rng.uniform(size=m) < 0.3
+    rl, ru = ax.copy(), ax.copy()
+    rl[rng_rows] -= rng.uniform(0.1, 1.0, rng_rows.sum())
+    ru[rng_rows] += rng.uniform(0.1, 1.0, rng_rows.sum())
+    return RawLP(c=c, A=A, rl=rl, ru=ru, cl=cl, cu=cu, name=f"mixed_{m}x{n}_s{seed}")
