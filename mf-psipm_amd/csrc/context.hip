@@ -1217,6 +1217,143 @@ k_pcg_direction2(int m, const double* __restrict__ partC, int nbC, int mode0, in
   }
 }
 
+// ---- device-judged two-column loop (as k_pcg_step_d / k_pcg_judge /
+// k_pcg_direction_d, per column; the hold flag lives in column 0's block)
+__global__ void __launch_bounds__(kBlock)
+k_pcg_step2_d(int m, const double* __restrict__ hv, const double* __restrict__ rzs, int cap,
+              const double* __restrict__ p, const double* __restrict__ q, double* __restrict__ x,
+              double* __restrict__ r, double* __restrict__ scal, double* __restrict__ alphas) {
+  const bool hold = scal[kHold] != 0.0;
+  double alpha[2] = {0.0, 0.0}, rzv[2] = {0.0, 0.0};
+  bool live[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    double* h = scal + c * kScalarHeader;
+    live[c] = !hold && h[kStop] == 0.0;
+    if (!live[c]) continue;
+    const double curvature = hv[3 * c];
+    const double floor_ = std::numeric_limits<double>::min() * fmax(1.0, sqrt(hv[3 * c + 1]) * sqrt(hv[3 * c + 2]));
+    rzv[c] = rzs[(int64_t)c * cap + (int)h[kRzSlot]];
+    const double a = rzv[c] / curvature;
+    int fail = 0;
+    if (!isfinite(curvature) || curvature <= floor_) fail = -2;
+    else if (!isfinite(a) || a <= 0.0) fail = -3;
+    if (fail) {
+      live[c] = false;
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        h[kInfo] = fail;
+        h[kCurv] = curvature;
+      }
+    }
+    alpha[c] = a;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0 && !hold) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      double* h = scal + c * kScalarHeader;
+      if (h[kStop] == 0.0 && !live[c]) h[kStop] = 1.0;
+    }
+  }
+  if (!live[0] && !live[1]) return;  // uniform per block
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (!live[c]) continue;
+      const int64_t o = 2 * i + c;
+      x[o] += alpha[c] * p[o];
+      const double ri = r[o] - alpha[c] * q[o];
+      r[o] = ri;
+      acc[c] += ri * ri;
+    }
+  }
+  block_sum<2>(acc);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) scal[2 * kScalarHeader + c * gridDim.x + blockIdx.x] = acc[c];
+    if (blockIdx.x == 0) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        if (!live[c]) continue;
+        double* h = scal + c * kScalarHeader;
+        h[kAlpha] = alpha[c];
+        h[kCurv] = hv[3 * c];
+        h[kRzUsed] = rzv[c];
+        alphas[(int64_t)c * cap + (int)h[kIt]] = alpha[c];
+      }
+    }
+  }
+}
+
+__global__ void k_pcg_judge2(int nb, double* __restrict__ scal) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  if (scal[kHold] != 0.0) return;
+  bool any = false;
+  for (int c = 0; c < 2; ++c) {
+    double* h = scal + c * kScalarHeader;
+    if (h[kStop] != 0.0) continue;
+    double rr = 0.0;
+    for (int b = 0; b < nb; ++b) rr += scal[2 * kScalarHeader + c * nb + b];  // the host's fixed order
+    const int it = (int)h[kIt] + 1;
+    h[kIt] = it;
+    h[kRR] = rr;
+    const int repl = (int)h[kRepl];
+    const double rec = sqrt(rr) / h[kScale];
+    const bool need = (repl > 0 && it % repl == 0) || rec <= h[kTol] || it >= (int)h[kMaxit];
+    h[kNeed] = need ? 1.0 : 0.0;
+    any = any || need;
+  }
+  if (any) scal[kHold] = 1.0;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_pcg_direction2_d(int m, const double* __restrict__ partC, int nbC, double* __restrict__ rzs, int cap,
+                   const double* __restrict__ z, double* __restrict__ p, double* __restrict__ scal,
+                   double* __restrict__ betas) {
+  if (scal[kHold] != 0.0) return;
+  double tot[2];
+  reduce_partials<2>(partC, nbC, tot);
+  double beta[2] = {0.0, 0.0};
+  bool live[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    double* h = scal + c * kScalarHeader;
+    live[c] = h[kStop] == 0.0;
+    if (!live[c]) continue;
+    const double rn = tot[c];
+    beta[c] = rn / h[kRzUsed];
+    int fail = 0;
+    if (!isfinite(rn) || rn <= 0.0) fail = -6;
+    else if (!isfinite(beta[c]) || beta[c] < 0.0) fail = -7;
+    if (fail) {
+      live[c] = false;
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        h[kInfo] = fail;
+        h[kStop] = 1.0;
+      }
+      continue;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      const int it = (int)h[kIt];
+      rzs[(int64_t)c * cap + it] = rn;
+      h[kRzSlot] = it;
+      const int nbeta = (int)h[kNBeta];
+      betas[(int64_t)c * cap + nbeta] = beta[c];
+      h[kNBeta] = nbeta + 1;
+      h[kBeta] = beta[c];
+    }
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (!live[c]) continue;
+      const int64_t o = 2 * i + c;
+      p[o] = z[o] + beta[c] * p[o];
+    }
+  }
+}
+
 static Block2& block2(psg_ctx* c) {
   if (!c->b2) {
     auto* b = new Block2;
@@ -1326,7 +1463,7 @@ int psg_pcg2(psg_ctx* c, const double* tol, int32_t maxit, int32_t replacement, 
     factor_apply2(c, b.r.p, b.z.p);
     nb = device_dot2(c, b.r.p, b.z.p, b.partD.p);
     read_partials(c, b.partD.p, nb, 2, tmp);
-    int info[2], it[2] = {0, 0}, rz_slot[2] = {0, 0}, alpha_start[2] = {0, 0}, beta_start[2] = {0, 0};
+    int info[2], it[2] = {0, 0}, alpha_start[2] = {0, 0}, beta_start[2] = {0, 0};
     int replacements[2] = {0, 0}, checks[2] = {0, 0};
     bool done[2] = {false, false};
     for (int q = 0; q < 2; ++q) {
@@ -1345,28 +1482,63 @@ int psg_pcg2(psg_ctx* c, const double* tol, int32_t maxit, int32_t replacement, 
       HIP_LAUNCH_CHECK();
       done[q] = true;
     };
+    auto sset = [&](int q, int k, double v) {
+      hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, b.scal.p + q * kScalarHeader + k, v);
+      HIP_LAUNCH_CHECK();
+    };
+    for (int q = 0; q < 2; ++q) {
+      sset(q, kIt, 0.0);
+      sset(q, kRzSlot, 0.0);
+      sset(q, kScale, scale[q]);
+      sset(q, kTol, tolq[q]);
+      sset(q, kRepl, (double)replacement);
+      sset(q, kMaxit, (double)maxit);
+      sset(q, kNeed, 0.0);
+    }
+    sset(0, kHold, 0.0);
     for (int q = 0; q < 2; ++q)
       if (done[q]) freeze(q);
-    while (!(done[0] && done[1])) {
+    // one iteration of both columns, judged on the device (no-ops under the hold)
+    auto iteration = [&]() {
       device_Hv2(c, b, b.p.p, b.q.p, b.partA.p, true, b.hvdots.p);
-      hipLaunchKernelGGL(k_pcg_step2, dim3(grid_m), dim3(kBlock), 0, s, m, b.hvdots.p, rzs[0] + rz_slot[0],
-                         rzs[1] + rz_slot[1], b.p.p, b.q.p, b.x.p, b.r.p, b.scal.p, als[0] + it[0], als[1] + it[1]);
+      hipLaunchKernelGGL(k_pcg_step2_d, dim3(grid_m), dim3(kBlock), 0, s, m, b.hvdots.p, b.rzs.p, cap, b.p.p, b.q.p,
+                         b.x.p, b.r.p, b.scal.p, b.alphas.p);
+      hipLaunchKernelGGL(k_pcg_judge2, dim3(1), dim3(1), 0, s, grid_m, b.scal.p);
+      factor_apply2(c, b.r.p, b.z.p);
+      const int nbC = device_dot2(c, b.r.p, b.z.p, b.partC.p);
+      hipLaunchKernelGGL(k_pcg_direction2_d, dim3(grid_m), dim3(kBlock), 0, s, m, b.partC.p, nbC, b.rzs.p, cap,
+                         b.z.p, b.p.p, b.scal.p, b.betas.p);
       HIP_LAUNCH_CHECK();
-      HIP_CHECK(hipMemcpyAsync(b.hscal, b.scal.p, (2 * kScalarHeader + 2 * grid_m) * sizeof(double),
-                               hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-      bool need_check[2] = {false, false};
+    };
+    const int batch = std::max(1, c->pcg_batch);
+    while (!(done[0] && done[1])) {
+      // never run past the next replacement boundary or the limit of a live column
+      int k = batch;
       for (int q = 0; q < 2; ++q) {
         if (done[q]) continue;
+        k = std::min(k, maxit - it[q]);
+        if (replacement > 0) k = std::min(k, replacement - it[q] % replacement);
+      }
+      k = std::max(k, 1);
+      for (int j = 0; j < k; ++j) iteration();
+      HIP_CHECK(hipMemcpyAsync(b.hscal, b.scal.p, 2 * kScalarHeader * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      for (int q = 0; q < 2; ++q) {
         const double* h = b.hscal + q * kScalarHeader;
-        if (h[kStop] != 0.0) {
+        it[q] = (int)h[kIt];
+        if (!done[q] && h[kStop] != 0.0) {
           info[q] = (int)h[kInfo];
           done[q] = true;
-          continue;
         }
-        ++it[q];
-        const double rr = host_sum(b.hscal + 2 * kScalarHeader + q * grid_m, grid_m);
-        const double rec = std::sqrt(rr) / scale[q];
+      }
+      if (done[0] && done[1]) break;
+      if (b.hscal[kHold] == 0.0) continue;
+      // held after the step of the current iteration: act per column as the
+      // host-judged loop did
+      bool need_check[2] = {false, false};
+      for (int q = 0; q < 2; ++q) {
+        if (done[q] || b.hscal[q * kScalarHeader + kNeed] == 0.0) continue;
+        const double rec = std::sqrt(b.hscal[q * kScalarHeader + kRR]) / scale[q];
         const bool replace = replacement > 0 && it[q] % replacement == 0;
         need_check[q] = replace || rec <= tolq[q];
       }
@@ -1403,21 +1575,23 @@ int psg_pcg2(psg_ctx* c, const double* tol, int32_t maxit, int32_t replacement, 
           }
         }
       }
+      for (int q = 0; q < 2; ++q)
+        if (!done[q] && it[q] >= maxit) freeze(q);  // iteration limit: info stays maxit (> 0)
       if (done[0] && done[1]) break;
-      for (int q = 0; q < 2; ++q) {
-        if (!done[q] && it[q] >= maxit) {
-          freeze(q);  // iteration limit: info stays maxit (> 0)
-        }
-      }
-      if (done[0] && done[1]) break;
+      // the direction update the hold skipped, for both live columns
+      int slot[2];
+      for (int q = 0; q < 2; ++q) slot[q] = (int)b.hscal[q * kScalarHeader + kRzSlot];
       factor_apply2(c, b.r.p, b.z.p);
       const int nbC = device_dot2(c, b.r.p, b.z.p, b.partC.p);
       hipLaunchKernelGGL(k_pcg_direction2, dim3(grid_m), dim3(kBlock), 0, s, m, b.partC.p, nbC, mode[0], mode[1],
-                         rzs[0] + rz_slot[0], rzs[1] + rz_slot[1], rzs[0] + it[0], rzs[1] + it[1], b.z.p, b.p.p,
+                         rzs[0] + slot[0], rzs[1] + slot[1], rzs[0] + it[0], rzs[1] + it[1], b.z.p, b.p.p,
                          b.scal.p, bes[0], bes[1]);
       HIP_LAUNCH_CHECK();
-      for (int q = 0; q < 2; ++q)
-        if (!done[q]) rz_slot[q] = it[q];
+      for (int q = 0; q < 2; ++q) {
+        if (!done[q]) sset(q, kRzSlot, (double)it[q]);
+        sset(q, kNeed, 0.0);
+      }
+      sset(0, kHold, 0.0);
     }
     // pick up direction-kernel failures raised after the last synchronisation
     HIP_CHECK(hipMemcpyAsync(b.hscal, b.scal.p, 2 * kScalarHeader * sizeof(double), hipMemcpyDeviceToHost, s));

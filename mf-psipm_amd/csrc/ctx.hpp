@@ -24,6 +24,7 @@ enum ScalarSlot {
   kTol = 12,
   kRepl = 13,    // residual replacement period (0: none)
   kMaxit = 14,
+  kNeed = 15,    // two-column loop: this column needs the host at the hold
   kScalarHeader = 16,  // partials of the last axpy follow the header
 };
 

@@ -115,7 +115,9 @@ def test_solve_pair_matches_single_solves_and_oracle(gpu, precision):
     g.solve(r1b, r2b)
     its_single = list(g.cg_iterations[-2:])
     if precision == "fp64":
-        assert all(abs(a - b) <= 1 for a, b in zip(its_pair, its_single)), (its_pair, its_single)
+        # same recurrences; the forward solve's atomic order moves tolerance
+        # crossings by an iteration or two
+        assert all(abs(a - b) <= 3 for a, b in zip(its_pair, its_single)), (its_pair, its_single)
     g.close()
 
 
@@ -136,3 +138,38 @@ def test_pc4_block_driver_matches_highs(gpu, precision):
     assert max(info["pinf"], info["dinf"]) <= 1e-8
     assert solver.pair_solves > 0
     solver.close()
+
+
+@pytest.mark.parametrize("replacement,maxit", [(100, 400), (9, 400), (0, 30)])
+def test_pair_launch_batch_does_not_change_the_outcome(gpu, replacement, maxit):
+    """The two-column loop is judged on the device too: 1, 4 or 16 iterations
+    between synchronisations give the same per-column outcome (up to the
+    forward solve's non-fixed atomic order)."""
+    _say(f"pair launch batch replacement={replacement} maxit={maxit}")
+    from mf_psipm_amd import autotune as AT
+    m, n = 1600, 4800
+    _, lp = _lp(m, n, 5, 41)
+    g = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, cg_tol=1e-10, cg_tol_max=1e-10,
+                                              cg_maxit=maxit, structural_backbone=False, core_factor=0.4)
+    g.residual_replacement = replacement
+    g.factorize(_theta(n, 42, 2.5), 1e-6)
+    rng = np.random.default_rng(43)
+    R1 = np.ascontiguousarray(np.stack([rng.standard_normal(n), np.zeros(n)], axis=1))
+    R2 = np.ascontiguousarray(np.stack([rng.standard_normal(m), 50.0 * rng.standard_normal(m)], axis=1))
+    outs = []
+    for batch in (1, 4, 16):
+        AT.set_(g, 5, batch)
+        g._call("psg_solve_begin2", g._ctx, R1, R2)
+        res = g._pcg_pair([1e-10, 1e-10])
+        dx, dy = np.empty((n, 2)), np.empty((m, 2))
+        g._call("psg_solve_finish2", g._ctx, dx, dy)
+        outs.append((res, dy))
+        _say(f"batch {batch}: " + " | ".join(f"info {r.info} it {r.iterations} checks {r.residual_checks} "
+                                              f"repl {r.replacements} rel {r.relative_residual:.2e}" for r in res))
+    r0, y0 = outs[0]
+    for res, dy in outs[1:]:
+        for q in range(2):
+            assert res[q].info == r0[q].info
+            assert abs(res[q].iterations - r0[q].iterations) <= 2
+            assert np.linalg.norm(dy[:, q] - y0[:, q]) <= 1e-7 * np.linalg.norm(y0[:, q])
+    g.close()
