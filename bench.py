@@ -93,6 +93,8 @@ def gpu_leg(lp, args, precision):
     iters = max(1, res.iterations)
     kt = np.zeros(4)
     solver._call("psg_time_kernels", solver._ctx, 20, kt)
+    finfo = np.zeros(8)
+    solver._call("psg_factor_info", solver._ctx, finfo)
     # two right-hand sides in one block PCG (every read of A and L serves both)
     pair = None
     if not args.no_pair:
@@ -114,7 +116,8 @@ def gpu_leg(lp, args, precision):
         factor_device_ms=solver.device_factor_ms, core=int(solver.core.size),
         nnz_L=int(solver.nnz_factor), plan=dict(solver._plan.info), hv_ms=kt[0], hv_loop_ms=kt[1],
         apply_ms=kt[2], dot_ms=kt[3], theta=theta, r2=r2, core_cols=solver.core.copy(), pair=pair,
-        tuned=tuned,
+        tuned=tuned, tail=dict(r=int(finfo[3]), supernodes=int(finfo[4]), symmetric=int(finfo[6]),
+                               panel_entries=int(finfo[7])),
     )
     solver.close()
     return out
@@ -328,9 +331,15 @@ def main():
             traffic = t
     bytes_hv = hv_bytes(m, n, nnz, 8)
     achieved = bytes_hv / (g64["hv_ms"] * 1e-3) / 1e9
-    # dominant kernel pair: the two single-launch triangular solves read every
-    # stored panel entry once each (8 B) plus the permuted vectors
-    bytes_apply = 2 * g64["plan"]["nnz_panels"] * 8 + 2 * g64["plan"]["row_structure"] * 4 + 8 * 8 * m
+    # dominant kernel chain P^-1 r: the two single-launch triangular solves
+    # read every stored panel entry below the dense tail once each (8 B), the
+    # tail is read once as the lower triangle of (L_T L_T')^-1 (or twice as
+    # L_T^-1 with the two-product tail), plus row indices and the permuted vectors
+    tail = g64["tail"]
+    tr = tail["r"]
+    tail_reads = (1 if tail["symmetric"] else 2) * (tr * (tr + 1) // 2) * 8
+    bytes_apply = (2 * (g64["plan"]["nnz_panels"] - tail["panel_entries"]) * 8 + tail_reads
+                   + 2 * g64["plan"]["row_structure"] * 4 + 8 * 8 * m)
     achieved_apply = bytes_apply / (g64["apply_ms"] * 1e-3) / 1e9
     line = {
         "metric": "PCG iterations/s (plain FP64 GPU path), 1 MI355X",
@@ -348,13 +357,14 @@ def main():
         "config": {"workload": f"random LP m={m} n={n} nnz/col={args.nnz_per_col} seed={args.seed}",
                    "theta": "10^U(-4,4) seeded, delta 1e-6", "core": g64["core"],
                    "nnz_L": g64["nnz_L"], "fill_budget": args.fill_budget,
-                   "supernodes": g64["plan"]["nsuper"], "levels": g64["plan"]["nlevels"]},
+                   "supernodes": g64["plan"]["nsuper"], "levels": g64["plan"]["nlevels"],
+                   "dense_tail": g64["tail"]},
         "factor_s": round(g64["factor_s"], 4),
         "mixed_it_per_s": None if mixed is None else round(mixed["it_per_s"], 3),
         "kernel_ms": {"Hv": g64["hv_ms"], "apply_P": g64["apply_ms"], "dot": g64["dot_ms"],
                       "Hv_mixed": None if mixed is None else mixed["hv_loop_ms"],
                       "apply_P_mixed": None if mixed is None else mixed["apply_ms"]},
-        "roofline": {"bound": "hbm", "kernel": "P^-1 r: k_fwd_low + k_fwd_sf + k_tail_lower + k_tail_upper + k_bwd_sf + k_bwd_low (triangular solves)",
+        "roofline": {"bound": "hbm", "kernel": "P^-1 r: k_fwd_low + k_fwd_sf + k_tail_symv + k_bwd_sf + k_bwd_low (triangular solves around the dense tail's symmetric product)",
                      "achieved": round(achieved_apply, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_apply / HBM_PEAK_GBS, 4), "traffic": traffic.get("apply_P_bytes"),
                      "bytes_per_launch": int(bytes_apply), "ms_per_launch": round(g64["apply_ms"], 4)},

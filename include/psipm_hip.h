@@ -246,9 +246,12 @@ int psg_solve_trace(psg_ctx* ctx, int64_t* out, int64_t cap, int64_t* nunits);
  * column of a non-positive pivot.  No counterpart in the reference (CHOLMOD
  * does the whole factor there, PS:855-885); used to test those kernels. */
 int psg_dense_cholinv(psg_ctx* ctx, int64_t n, double* a);
-/* Diagnostics of the factor's solve layout, out[6] = {solve mode (0 level
+/* Diagnostics of the factor's solve layout, out[8] = {solve mode (0 level
  * scheduled, 1 single launch), dependency-driven units, isolated 1x1
- * supernodes, dense-tail order r (0: none), dense-tail supernodes, nsuper}.
+ * supernodes, dense-tail order r (0: none), dense-tail supernodes, nsuper,
+ * dense-tail apply (1: one symmetric product with (L_T L_T')^-1, 0: two
+ * triangular products with L_T^-1, or no tail), panel entries of the
+ * dense-tail supernodes}.
  * No counterpart in the reference. */
 int psg_factor_info(psg_ctx* ctx, double* out);
 /* ---- learned column selector ---------------------------------------------

@@ -364,11 +364,26 @@ k_pcg_step_d(int m, const double* __restrict__ hv_dots, const double* __restrict
 
 // one thread: count the step, keep |r|^2 and hold for the host when the
 // recurrence meets the tolerance, a replacement is due or maxit is reached
-__global__ void k_pcg_judge(int nb, double* __restrict__ scal) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  if (scal[kStop] != 0.0 || scal[kHold] != 0.0) return;
+// Sum of the nb (<= kRedBlocks) block partials src[0..nb) in index order --
+// the host's fixed order -- by thread 0 of a one-wave block; the wave first
+// stages the partials in LDS with coalesced loads, so the sequential sum
+// waits on LDS rather than on nb dependent global loads.
+__device__ inline double ordered_partial_sum(const double* __restrict__ src, int nb) {
+  __shared__ double sp[kRedBlocks];
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) sp[b] = src[b];
+  __syncthreads();
   double rr = 0.0;
-  for (int b = 0; b < nb; ++b) rr += scal[kScalarHeader + b];  // the host's fixed order
+  if (threadIdx.x == 0)
+    for (int b = 0; b < nb; ++b) rr += sp[b];
+  __syncthreads();
+  return rr;
+}
+
+__global__ void k_pcg_judge(int nb, double* __restrict__ scal) {
+  if (blockIdx.x != 0) return;
+  if (scal[kStop] != 0.0 || scal[kHold] != 0.0) return;  // uniform over the block
+  const double rr = ordered_partial_sum(scal + kScalarHeader, nb);
+  if (threadIdx.x != 0) return;
   const int it = (int)scal[kIt] + 1;
   scal[kIt] = it;
   scal[kRR] = rr;
@@ -869,7 +884,7 @@ int psg_pcg(psg_ctx* c, double tol, int32_t maxit, int32_t replacement, double g
       device_Hv(c, c->p.p, c->q.p, c->partA.p, true, c->hvdots.p);
       hipLaunchKernelGGL(k_pcg_step_d, dim3(grid_m), dim3(kBlock), 0, s, m, c->hvdots.p, c->rzs.p, c->p.p, c->q.p,
                          c->x.p, c->r.p, c->scal.p, c->alphas.p);
-      hipLaunchKernelGGL(k_pcg_judge, dim3(1), dim3(1), 0, s, grid_m, c->scal.p);
+      hipLaunchKernelGGL(k_pcg_judge, dim3(1), dim3(kWave), 0, s, grid_m, c->scal.p);
       factor_apply(c, c->r.p, c->z.p);
       const int nbC = device_dot(c, c->r.p, c->z.p, m, c->partC.p);
       hipLaunchKernelGGL(k_pcg_direction_d, dim3(grid_m), dim3(kBlock), 0, s, m, c->partC.p, nbC, c->rzs.p, c->z.p,
@@ -1287,14 +1302,16 @@ k_pcg_step2_d(int m, const double* __restrict__ hv, const double* __restrict__ r
 }
 
 __global__ void k_pcg_judge2(int nb, double* __restrict__ scal) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  if (scal[kHold] != 0.0) return;
+  if (blockIdx.x != 0) return;
+  if (scal[kHold] != 0.0) return;  // uniform over the block
+  double rrs[2];
+  for (int c = 0; c < 2; ++c) rrs[c] = ordered_partial_sum(scal + 2 * kScalarHeader + c * nb, nb);
+  if (threadIdx.x != 0) return;
   bool any = false;
   for (int c = 0; c < 2; ++c) {
     double* h = scal + c * kScalarHeader;
     if (h[kStop] != 0.0) continue;
-    double rr = 0.0;
-    for (int b = 0; b < nb; ++b) rr += scal[2 * kScalarHeader + c * nb + b];  // the host's fixed order
+    const double rr = rrs[c];
     const int it = (int)h[kIt] + 1;
     h[kIt] = it;
     h[kRR] = rr;
@@ -1503,7 +1520,7 @@ int psg_pcg2(psg_ctx* c, const double* tol, int32_t maxit, int32_t replacement, 
       device_Hv2(c, b, b.p.p, b.q.p, b.partA.p, true, b.hvdots.p);
       hipLaunchKernelGGL(k_pcg_step2_d, dim3(grid_m), dim3(kBlock), 0, s, m, b.hvdots.p, b.rzs.p, cap, b.p.p, b.q.p,
                          b.x.p, b.r.p, b.scal.p, b.alphas.p);
-      hipLaunchKernelGGL(k_pcg_judge2, dim3(1), dim3(1), 0, s, grid_m, b.scal.p);
+      hipLaunchKernelGGL(k_pcg_judge2, dim3(1), dim3(kWave), 0, s, grid_m, b.scal.p);
       factor_apply2(c, b.r.p, b.z.p);
       const int nbC = device_dot2(c, b.r.p, b.z.p, b.partC.p);
       hipLaunchKernelGGL(k_pcg_direction2_d, dim3(grid_m), dim3(kBlock), 0, s, m, b.partC.p, nbC, b.rzs.p, cap,

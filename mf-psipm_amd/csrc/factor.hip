@@ -35,6 +35,7 @@ namespace psg {
 constexpr int kW = 64;      // max supernode width handled by the panel kernels
 constexpr int kRB = 256;    // rows per TRSM / solve unit
 constexpr int kTile = 64;   // update tile edge
+constexpr int kSC = 64;     // max columns per symmetric dense-tail tile
 
 struct LevelRange {
   int node_begin, node_end;   // into level_nodes
@@ -53,7 +54,9 @@ struct MGemm {
   int transB;               // op(B) = B' with B stored N x K
   int lower;                // write only entries with row - col + diag >= 0
   int diag;
-  int tri;                  // 1: B lower triangular (k >= col), 2: A lower triangular (k <= row)
+  int tri;                  // 1: B lower triangular (k >= col), 2: A lower triangular (k <= row),
+                            // 3: both operands vanish for k < max(row, col)
+  int transA;               // op(A) = A' with A stored K x M
   double alpha, beta;
 };
 
@@ -88,11 +91,20 @@ struct FactorState {
   // factored and inverted as one dense block; its inverse X = L_T^-1
   // overwrites L_T in tail_L
   int tail_r = 0, tail_F = 0, tail_s0 = 0, tail_nchunk = 0;
+  int64_t tail_panel_entries = 0;  // panel entries of the tail supernodes
   DevBuf<int> tail_chunk_of, tail_lower_tiles, tail_upper_tiles;
   int tail_nlower = 0, tail_nupper = 0;
   int tail_tc = 512, tail_ur = 1024;  // lower-tile columns, upper-tile rows
   DevBuf<double> tail_L64, tail_S64;
   DevBuf<float> tail_L32, tail_S32;
+  // symmetric tail (tail_sym = 1, the FP64 default): Q = X' X in tail_Q
+  // (leading dimension tail_ld), applied by one product over tail_sym_tiles
+  // of tail_sc columns x (kBlock V tail_kv) rows; 0 keeps the two triangular
+  // products with X
+  int tail_sym = 1, tail_ld = 0, tail_sc = kSC, tail_kv = 4, tail_nsym = 0;
+  DevBuf<int> tail_sym_tiles;
+  DevBuf<double> tail_Q64;
+  DevBuf<float> tail_Q32;
   struct TailStep { int kind, j, w; MGemm g; };  // kind: 0 potrf, 1 trsm, 2 update
   std::vector<TailStep> tail_steps;
   struct InvLaunch { int tile_begin, ntile; };
@@ -901,6 +913,8 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
 // wide bottom levels (thousands of small supernodes) run at full occupancy
 // instead of one workgroup per unit.
 // ---------------------------------------------------------------------------
+constexpr int kCK = 8;  // loads a lane issues back to back in the bottom-level kernels
+
 template <class T, int NR = 1>
 __global__ void __launch_bounds__(kBlock)
 k_fwd_low(int count, const int* __restrict__ nodes, const int* __restrict__ sfirst, const int64_t* __restrict__ rptr,
@@ -914,6 +928,15 @@ k_fwd_low(int count, const int* __restrict__ nodes, const int* __restrict__ sfir
   const int w = sfirst[s + 1] - f;
   const int64_t nr = rptr[s + 1] - rptr[s];
   const T* D = dinv + doff[s];
+  const T* P = panels + poff[s];
+  const int* R = rows + rptr[s];
+  // the first panel chunk does not depend on b: its loads go out first, so
+  // the wave waits for memory about once per chunk rather than once per column
+  const int64_t i_first = w + lane;
+  T p_first[kCK];
+#pragma unroll
+  for (int c = 0; c < kCK; ++c) p_first[c] = (c < w && i_first < nr) ? P[(int64_t)c * nr + i_first] : T(0);
+  const int row_first = i_first < nr ? R[i_first] : 0;
   // y_s = L_ss^-1 b_s, lane j owns y_j
   T bj[NR], yj[NR];
 #pragma unroll
@@ -921,43 +944,45 @@ k_fwd_low(int count, const int* __restrict__ nodes, const int* __restrict__ sfir
     bj[q] = (lane < w) ? b[(int64_t)(f + lane) * NR + q] : T(0);
     yj[q] = T(0);
   }
-  for (int k = 0; k < w; ++k) {
-    const T dk = (lane < w) ? D[(int64_t)k * w + lane] : T(0);
+  for (int k0 = 0; k0 < w; k0 += kCK) {
+    T dv[kCK];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) yj[q] += dk * __shfl(bj[q], k);
+    for (int c = 0; c < kCK; ++c) dv[c] = (lane < w && k0 + c < w) ? D[(int64_t)(k0 + c) * w + lane] : T(0);
+#pragma unroll
+    for (int c = 0; c < kCK; ++c)
+#pragma unroll
+      for (int q = 0; q < NR; ++q) yj[q] += dv[c] * __shfl(bj[q], k0 + c);
   }
   if (lane < w) {
 #pragma unroll
     for (int q = 0; q < NR; ++q) y[(int64_t)(f + lane) * NR + q] = yj[q];
   }
   // b[rows below] -= L_below y_s, lane = row
-  const T* P = panels + poff[s];
-  const int* R = rows + rptr[s];
   for (int64_t i0 = w; i0 < nr; i0 += kWave) {
     const int64_t i = i0 + lane;
-    T acc0[NR], acc1[NR];
+    const bool first = i0 == w;
+    T acc[NR];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) acc0[q] = acc1[q] = T(0);
-    int c = 0;
-    for (; c + 1 < w; c += 2) {
-      const T p0 = (i < nr) ? P[(int64_t)c * nr + i] : T(0);
-      const T p1 = (i < nr) ? P[(int64_t)(c + 1) * nr + i] : T(0);
+    for (int q = 0; q < NR; ++q) acc[q] = T(0);
+    for (int c0 = 0; c0 < w; c0 += kCK) {
+      T pv[kCK];
+      if (first && c0 == 0) {
 #pragma unroll
-      for (int q = 0; q < NR; ++q) {
-        acc0[q] += p0 * __shfl(yj[q], c);
-        acc1[q] += p1 * __shfl(yj[q], c + 1);
+        for (int c = 0; c < kCK; ++c) pv[c] = p_first[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < kCK; ++c) pv[c] = (c0 + c < w && i < nr) ? P[(int64_t)(c0 + c) * nr + i] : T(0);
       }
-    }
-    if (c < w) {
-      const T p0 = (i < nr) ? P[(int64_t)c * nr + i] : T(0);
 #pragma unroll
-      for (int q = 0; q < NR; ++q) acc0[q] += p0 * __shfl(yj[q], c);
+      for (int c = 0; c < kCK; ++c)
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] += pv[c] * __shfl(yj[q], c0 + c);
     }
     if (i < nr) {
+      const int row = first ? row_first : R[i];
 #pragma unroll
       for (int q = 0; q < NR; ++q)
-        __hip_atomic_fetch_add(b + (int64_t)R[i] * NR + q, -(acc0[q] + acc1[q]), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(b + (int64_t)row * NR + q, -acc[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -1011,15 +1036,25 @@ k_bwd_low(int count, const int* __restrict__ nodes, const int* __restrict__ sfir
     const T red = transpose_reduce16(acc[q], lane);  // column cg + 4 rq
     t[q] = (c_own < w) ? y[(int64_t)(f + c_own) * NR + q] - red : T(0);
   }
-  // x_s = L_ss^-T t: lane j sums D(c, j) t_c over c >= j
+  // x_s = L_ss^-T t: lane j sums D(c, j) t_c over c >= j, kCK loads at a time
   T xj[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) xj[q] = T(0);
-  for (int c = 0; c < w; ++c) {
-    const int src = (c >> 2) + 16 * (c & 3);
-    const T dv = (lane < w && c >= lane) ? dinv[doff[s] + (int64_t)lane * w + c] : T(0);
+  const T* Drow = dinv + doff[s] + (int64_t)lane * w;
+  for (int c0 = 0; c0 < w; c0 += kCK) {
+    T dv[kCK];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) xj[q] += dv * __shfl(t[q], src);
+    for (int k = 0; k < kCK; ++k) {
+      const int c = c0 + k;
+      dv[k] = (lane < w && c < w && c >= lane) ? Drow[c] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < kCK; ++k) {
+      const int c = c0 + k;
+      const int src = (c >> 2) + 16 * (c & 3);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) xj[q] += dv[k] * __shfl(t[q], src);
+    }
   }
   if (lane < w) {
 #pragma unroll
@@ -1120,7 +1155,10 @@ k_mfma_gemm(MGemm single, int ntm, const MGemm* __restrict__ descs, const int* _
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int idx = tid + kBlock * e;
-      {  // A(i, k), consecutive lanes walk a column
+      if (d.transA) {  // op(A)(i, k) = A(k, i): consecutive lanes walk k down a column of A
+        const int gk = k0 + (idx & (kKT - 1)), gi = i0 + (idx >> 4);
+        ra[e] = (gi < d.M && gk < d.K) ? A[(int64_t)gi * d.lda + gk] : T(0);
+      } else {  // A(i, k), consecutive lanes walk a column
         const int gi = i0 + (idx & (kMT - 1)), gk = k0 + (idx >> 7);
         ra[e] = (gi < d.M && gk < d.K) ? A[(int64_t)gk * d.lda + gi] : T(0);
       }
@@ -1137,13 +1175,15 @@ k_mfma_gemm(MGemm single, int ntm, const MGemm* __restrict__ descs, const int* _
   int kbeg = 0, kend = d.K;
   if (d.tri == 1) kbeg = (j0 / kKT) * kKT;
   if (d.tri == 2) kend = min(d.K, i0 + kMT);
+  if (d.tri == 3) kbeg = (max(i0, j0) / kKT) * kKT;
   load(kbeg);
   for (int k0 = kbeg; k0 < kend; k0 += kKT) {
     __syncthreads();  // the previous chunk's LDS reads are done
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int idx = tid + kBlock * e;
-      sA[idx >> 7][idx & (kMT - 1)] = ra[e];
+      if (d.transA) sA[idx & (kKT - 1)][idx >> 4] = ra[e];
+      else sA[idx >> 7][idx & (kMT - 1)] = ra[e];
       if (d.transB) sB[idx >> 7][idx & (kMT - 1)] = rb[e];
       else sB[idx & (kKT - 1)][idx >> 4] = rb[e];
     }
@@ -1407,6 +1447,109 @@ k_tail_upper_vec(int r, int ur, const int* __restrict__ tiles, const T* __restri
   }
 }
 
+// x_T = Q b_T with Q = X' X = (L_T L_T')^-1, the two tail products folded
+// into one symmetric product that reads each stored entry once.  Q's lower
+// triangle is stored column-major with leading dimension ld (a multiple of the
+// 16-byte vector width).  Tile (J, I) covers columns [J sc, J sc + sc) and
+// rows [I RT, I RT + RT); wave w owns a contiguous quarter of the rows, each
+// lane KV vectors of V rows.  Entry (i, j) with i > j adds Q_ij b_j to x_i
+// (kept per lane over the tile's columns) and Q_ij b_i to x_j (summed over
+// the wave, then over the four waves in LDS); the diagonal adds once.
+template <class T, int NR, int KV>
+__global__ void __launch_bounds__(kBlock)
+k_tail_symv(int r, int ld, int sc, const int* __restrict__ tiles, const T* __restrict__ Q, const T* __restrict__ b,
+            T* __restrict__ x, int* __restrict__ done, int ndone) {
+  typedef typename Vec16<T>::v VT;
+  constexpr int V = Vec16<T>::n;
+  constexpr int RW = kWave * V * KV;         // rows per wave
+  constexpr int NW = kBlock / kWave;
+  constexpr int RT = RW * NW;                // rows per tile
+  __shared__ double bj_s[NR][kSC];
+  __shared__ double cpart[NW][NR][kSC];
+  const int J = tiles[2 * blockIdx.x], I = tiles[2 * blockIdx.x + 1];
+  const int j0 = J * sc, i0 = I * RT;
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  // the tail supernodes count as solved for the backward launch that follows
+  if (blockIdx.x == 0)
+    for (int q = threadIdx.x; q < ndone; q += blockDim.x) done[q] = 1;
+  for (int q = threadIdx.x; q < sc; q += blockDim.x) {
+#pragma unroll
+    for (int e = 0; e < NR; ++e) bj_s[e][q] = (j0 + q < r) ? (double)b[(int64_t)(j0 + q) * NR + e] : 0.0;
+  }
+  const int iw = i0 + wv * RW + lane * V;  // first row of this lane's vector k is iw + k kWave V
+  double bi[KV][V][NR], racc[KV][V][NR];
+#pragma unroll
+  for (int k = 0; k < KV; ++k)
+#pragma unroll
+    for (int t = 0; t < V; ++t) {
+      const int row = iw + k * kWave * V + t;
+#pragma unroll
+      for (int e = 0; e < NR; ++e) {
+        bi[k][t][e] = row < r ? (double)b[(int64_t)row * NR + e] : 0.0;
+        racc[k][t][e] = 0.0;
+      }
+    }
+  __syncthreads();
+  const int jend = min(j0 + sc, r);
+  for (int j = j0; j < jend; ++j) {
+    const T* col = Q + (int64_t)j * ld;
+    double bj[NR], cacc[NR];
+#pragma unroll
+    for (int e = 0; e < NR; ++e) {
+      bj[e] = bj_s[e][j - j0];
+      cacc[e] = 0.0;
+    }
+    VT v[KV];
+#pragma unroll
+    for (int k = 0; k < KV; ++k) {
+      const int i = iw + k * kWave * V;
+      if (i + V > j && i < r) v[k] = *reinterpret_cast<const VT*>(col + i);
+    }
+#pragma unroll
+    for (int k = 0; k < KV; ++k) {
+      const int i = iw + k * kWave * V;
+      if (!(i + V > j && i < r)) continue;
+#pragma unroll
+      for (int t = 0; t < V; ++t) {
+        const int row = i + t;
+        const double q = (row >= j && row < r) ? (double)v[k][t] : 0.0;
+        const double qo = row > j ? q : 0.0;
+#pragma unroll
+        for (int e = 0; e < NR; ++e) {
+          racc[k][t][e] += q * bj[e];
+          cacc[e] += qo * bi[k][t][e];
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < NR; ++e) {
+      const double sm = wave_sum(cacc[e]);
+      if (lane == 0) cpart[wv][e][j - j0] = sm;
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < sc * NR; q += blockDim.x) {
+    const int e = q / sc, cc = q - e * sc;
+    if (j0 + cc >= r) continue;
+    double sm = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) sm += cpart[w][e][cc];
+    __hip_atomic_fetch_add(x + (int64_t)(j0 + cc) * NR + e, (T)sm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+#pragma unroll
+  for (int k = 0; k < KV; ++k)
+#pragma unroll
+    for (int t = 0; t < V; ++t) {
+      const int row = iw + k * kWave * V + t;
+      if (row < r && row >= j0) {
+#pragma unroll
+        for (int e = 0; e < NR; ++e)
+          __hip_atomic_fetch_add(x + (int64_t)row * NR + e, (T)racc[k][t][e], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+}
+
 __global__ void k_fill_int(int* __restrict__ p, int n, int v) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
@@ -1450,6 +1593,48 @@ static void setup_tail(psg_ctx* c, const Plan& P) {
   fs->tail_lower_tiles.upload(lt, st);
   fs->tail_upper_tiles.upload(ut, st);
   setup_dense_schedule(fs, r, c->precision, st);
+  // symmetric product: the widest tiles that still give ~1000 workgroups
+  // (columns sc in {16, 32, 64}, vectors per lane kv in {1, 2, 4})
+  // The explicit Q = X' X keeps its positive definiteness only while
+  // cond(L_T L_T') stays well below 1 / (unit roundoff): always for FP64 at
+  // the shifts the ladder uses, but not for FP32, where x = X' (X b) keeps
+  // b' x = |X b|^2 >= 0 by construction.  PSG_TAIL_SYM=0/1 overrides.
+  const char* sym = getenv("PSG_TAIL_SYM");
+  fs->tail_sym = sym ? (sym[0] == '0' ? 0 : 1) : (c->precision == PSG_MIXED ? 0 : 1);
+  if (fs->tail_sym) {
+    const int V = c->precision == PSG_MIXED ? Vec16<float>::n : Vec16<double>::n;
+    fs->tail_ld = (r + 3) / 4 * 4;
+    int sc = kSC, kv = V == 4 ? 2 : 4;  // 2048-row tiles (FP32: register budget)
+    auto ntiles = [&](int sc_, int kv_) {
+      const int rt = kBlock * V * kv_;
+      int64_t n = 0;
+      for (int J = 0; J * sc_ < r; ++J) n += (r + rt - 1) / rt - (J * sc_) / rt;
+      return n;
+    };
+    while (ntiles(sc, kv) < 1024 && (kv > 1 || sc > 16)) {
+      if (kv > 1) kv >>= 1;
+      else sc >>= 1;
+    }
+    fs->tail_sc = sc;
+    fs->tail_kv = kv;
+    const int rt = kBlock * V * kv;
+    std::vector<int> qt;
+    for (int J = 0; J * sc < r; ++J)
+      for (int I = (J * sc) / rt; I * rt < r; ++I) { qt.push_back(J); qt.push_back(I); }
+    fs->tail_nsym = (int)qt.size() / 2;
+    fs->tail_sym_tiles.upload(qt, st);
+    const size_t nq = (size_t)fs->tail_ld * r;
+    if (c->precision == PSG_MIXED) {
+      fs->tail_Q32.alloc(nq);
+      fs->tail_Q64.release();
+    } else {
+      fs->tail_Q64.alloc(nq);
+      fs->tail_Q32.release();
+    }
+  } else {
+    fs->tail_Q64.release();
+    fs->tail_Q32.release();
+  }
 }
 
 // Blocked Cholesky and inversion schedule of an r x r dense block, and its
@@ -1562,7 +1747,7 @@ static void dense_factor_inverse(FactorState* fs, hipStream_t st, int r, int F, 
 // rest of the tree.  Gather them into L (r x r), factor it, and overwrite it
 // with X = L^-1.
 template <class T>
-static void tail_factor_dense(psg_ctx* c, const T* panels, T* L, T* Sc) {
+static void tail_factor_dense(psg_ctx* c, const T* panels, T* L, T* Sc, T* Q) {
   FactorState* fs = c->fs;
   hipStream_t st = c->stream;
   const int r = fs->tail_r, F = fs->tail_F;
@@ -1572,6 +1757,21 @@ static void tail_factor_dense(psg_ctx* c, const T* panels, T* L, T* Sc) {
                      fs->rptr.p, fs->rows.p, fs->poff.p, fs->tail_chunk_of.p, panels, L);
   HIP_LAUNCH_CHECK();
   dense_factor_inverse<T>(fs, st, r, F, L, Sc);
+  if (fs->tail_sym) {
+    // lower triangle of Q = X' X; X(k, i) vanishes for k < i, so the sum
+    // over k starts at max(row, col)
+    MGemm g{};
+    g.M = r; g.N = r; g.K = r;
+    g.a_sel = 0; g.a_off = 0; g.lda = r; g.transA = 1;
+    g.b_sel = 0; g.b_off = 0; g.ldb = r;
+    g.c_sel = 1; g.c_off = 0; g.ldc = fs->tail_ld;
+    g.lower = 1; g.diag = 0; g.tri = 3;
+    g.alpha = 1.0; g.beta = 0.0;
+    const int ntm = (r + kMT - 1) / kMT;
+    hipLaunchKernelGGL((k_mfma_gemm<T>), dim3(mgemm_blocks(g)), dim3(kBlock), 0, st, g, ntm, (const MGemm*)nullptr,
+                       (const int*)nullptr, (const int*)nullptr, L, Q);
+    HIP_LAUNCH_CHECK();
+  }
 }
 
 // Cholesky of the r x r block in L (lower part; the upper part must be zero)
@@ -1601,10 +1801,28 @@ static void dense_factor_inverse(FactorState* fs, hipStream_t st, int r, int F, 
 }
 
 template <class T, int NR = 1>
-static void tail_solve(psg_ctx* c, const T* X, T* vb, T* vy, T* vx, bool clean) {
+static void tail_solve(psg_ctx* c, const T* X, const T* Q, T* vb, T* vy, T* vx, bool clean) {
   FactorState* fs = c->fs;
   hipStream_t st = c->stream;
   const int r = fs->tail_r, F = fs->tail_F;
+  if (fs->tail_sym) {
+    if (!clean) zero_async(vx + (size_t)F * NR, (int64_t)r * NR, st);
+    const T* bt = vb + (size_t)F * NR;
+    T* xt = vx + (size_t)F * NR;
+    const int ld = fs->tail_ld, sc = fs->tail_sc;
+    const int* tl = fs->tail_sym_tiles.p;
+    const dim3 grid(fs->tail_nsym), blk(kBlock);
+    int* done = fs->flags.p + 3 * fs->nsuper + fs->tail_s0;
+    const int nd = fs->tail_nchunk;
+    if (fs->tail_kv == 4)
+      hipLaunchKernelGGL((k_tail_symv<T, NR, 4>), grid, blk, 0, st, r, ld, sc, tl, Q, bt, xt, done, nd);
+    else if (fs->tail_kv == 2)
+      hipLaunchKernelGGL((k_tail_symv<T, NR, 2>), grid, blk, 0, st, r, ld, sc, tl, Q, bt, xt, done, nd);
+    else
+      hipLaunchKernelGGL((k_tail_symv<T, NR, 1>), grid, blk, 0, st, r, ld, sc, tl, Q, bt, xt, done, nd);
+    HIP_LAUNCH_CHECK();
+    return;
+  }
   if (!clean) zero_async(vy + (size_t)F * NR, (int64_t)r * NR, st);
   hipLaunchKernelGGL((k_tail_lower<T, NR>), dim3(fs->tail_nlower), dim3(kBlock), 0, st, r, fs->tail_tc,
                      fs->tail_lower_tiles.p, X,
@@ -1692,6 +1910,11 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
       fs->tail_nchunk = best_q - best_s0 + 1;
       fs->tail_F = P.sfirst[best_s0];
     }
+  }
+  fs->tail_panel_entries = 0;
+  for (int k = 0; k < fs->tail_nchunk; ++k) {
+    const int sn = fs->tail_s0 + k;
+    fs->tail_panel_entries += (P.rptr[sn + 1] - P.rptr[sn]) * (int64_t)(P.sfirst[sn + 1] - P.sfirst[sn]);
   }
   auto in_tail = [&](int sn) { return fs->tail_nchunk && sn >= fs->tail_s0 && sn < fs->tail_s0 + fs->tail_nchunk; };
   // unit and tile lists per level; the tail's supernodes are factored as
@@ -1978,9 +2201,9 @@ static int factor_run(psg_ctx* c, const Plan& P, double eta, T* panels, T* dinv)
   }
   if (fs->tail_nchunk) {
     if constexpr (sizeof(T) == 8)
-      tail_factor_dense<double>(c, panels, fs->tail_L64.p, fs->tail_S64.p);
+      tail_factor_dense<double>(c, panels, fs->tail_L64.p, fs->tail_S64.p, fs->tail_Q64.p);
     else
-      tail_factor_dense<float>(c, panels, fs->tail_L32.p, fs->tail_S32.p);
+      tail_factor_dense<float>(c, panels, fs->tail_L32.p, fs->tail_S32.p, fs->tail_Q32.p);
   }
   int failcol = INT_MAX;
   HIP_CHECK(hipMemcpyAsync(&failcol, fs->fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -2110,9 +2333,9 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   }
   if (fs->tail_nchunk) {
     if constexpr (sizeof(T) == 8)
-      tail_solve<double, NR>(c, fs->tail_L64.p, vb, vy, vx, clean);
+      tail_solve<double, NR>(c, fs->tail_L64.p, fs->tail_Q64.p, vb, vy, vx, clean);
     else
-      tail_solve<float, NR>(c, fs->tail_L32.p, vb, vy, vx, clean);
+      tail_solve<float, NR>(c, fs->tail_L32.p, fs->tail_Q32.p, vb, vy, vx, clean);
   }
   if (fs->sf_units) {
     hipLaunchKernelGGL((k_bwd_sf<T, NR>), dim3(grid), dim3(kBlock), 0, s, a);
@@ -2281,6 +2504,8 @@ extern "C" int psg_factor_info(psg_ctx* c, double* out) {
     out[3] = fs->tail_r;
     out[4] = fs->tail_nchunk;
     out[5] = fs->nsuper;
+    out[6] = fs->tail_nchunk ? fs->tail_sym : 0;
+    out[7] = (double)fs->tail_panel_entries;
     return PSG_OK;
   });
 }

@@ -49,6 +49,11 @@ struct MinDegree {
   std::vector<char> eliminated;
   std::vector<int> mark;                    // scratch marks
   std::vector<int> wext;                    // |L_e \ Lp| scratch
+  // total weight of the live variables of element e.  It is fixed once the
+  // element exists: a variable leaves L_e only by being pivoted (which
+  // absorbs e) and a supervariable merge moves weight between two variables
+  // that belong to exactly the same elements.
+  std::vector<int> elem_w;
   int stamp = 0;
   int min_deg = 0;
 
@@ -108,6 +113,7 @@ std::vector<int> minimum_degree_order(int m, int ncols, const int64_t* colptr,
   g.eliminated.assign(m, 0);
   g.mark.assign(std::max<size_t>((size_t)m, (size_t)ncols + m), 0);
   g.wext.assign((size_t)ncols + m, -1);
+  g.elem_w.assign((size_t)ncols + m, 0);
 
   // initial (over-)estimate of the external degree: sum of clique sizes,
   // capped by m - 1; exact degrees arrive as soon as a neighbour is pivoted.
@@ -142,6 +148,8 @@ std::vector<int> minimum_degree_order(int m, int ncols, const int64_t* colptr,
       L.erase(std::remove_if(L.begin(), L.end(), [&](int v) { return g.weight[v] == 0; }), L.end());
     }
   }
+  for (size_t e = 0; e < (size_t)ncols; ++e)
+    if (g.elem_alive[e]) g.elem_w[e] = (int)g.vars_of[e].size();
 
   std::vector<int> Lp;
   std::vector<int> touched_elems;
@@ -174,6 +182,7 @@ std::vector<int> minimum_degree_order(int m, int ncols, const int64_t* colptr,
     const int pe = ncols + p;
     g.vars_of[pe] = Lp;
     g.elem_alive[pe] = 1;
+    g.elem_w[pe] = lp_weight;
     std::vector<int>().swap(g.elems_of[p]);
 
     // ---- update element lists of i in Lp, compute |L_e \ Lp| ---------------
@@ -186,9 +195,7 @@ std::vector<int> minimum_degree_order(int m, int ncols, const int64_t* colptr,
       E.resize(k);
       for (int e : E) {
         if (g.wext[e] < 0) {
-          int wsum = 0;
-          for (int v : g.vars_of[e]) if (g.weight[v] > 0 && !g.eliminated[v]) wsum += g.weight[v];
-          g.wext[e] = wsum;
+          g.wext[e] = g.elem_w[e];
           touched_elems.push_back(e);
         }
         g.wext[e] -= g.weight[i];
@@ -232,7 +239,7 @@ std::vector<int> minimum_degree_order(int m, int ncols, const int64_t* colptr,
     for (size_t a = 0; a < hashed.size();) {
       size_t b = a + 1;
       while (b < hashed.size() && hashed[b].first == hashed[a].first) ++b;
-      for (size_t x = a; x < b; ++x) {
+      for (size_t x = a; x + 1 < b; ++x) {  // a lone hash has no twin
         int i = hashed[x].second;
         if (g.weight[i] == 0) continue;
         auto& Ei = g.elems_of[i];

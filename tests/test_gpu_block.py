@@ -68,7 +68,7 @@ def test_apply_P2_with_dense_tail(gpu):
     _, lp = _lp(m, n, 6, 8)
     obj = HipSpectralStableColumnPreconditioner(lp.A, protected=~lp.pos, core_factor=0.6, structural_backbone=False)
     obj.factorize(_theta(n, 9, 1.0), 1e-4)
-    info = np.zeros(6)
+    info = np.zeros(8)
     obj._call("psg_factor_info", obj._ctx, info)
     _say(f"factor layout {info.tolist()}")
     rng = np.random.default_rng(10)

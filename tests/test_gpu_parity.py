@@ -277,13 +277,16 @@ def test_dense_tail_inverse_solve(gpu, monkeypatch, precision, m, n, k, frac, ta
     theta = _theta(n, 73, spread=2.0)
     monkeypatch.setenv("PSG_TAIL_MIN", str(tail_min))
     outs, infos = [], []
-    for mode in ("level", "single"):
+    # level-scheduled substitution; single launch with the symmetric tail
+    # product (L_T L_T')^-1 b; single launch with the two triangular products
+    for mode, sym in (("level", "1"), ("single", "1"), ("single", "0")):
         monkeypatch.setenv("PSG_SOLVE", mode)
+        monkeypatch.setenv("PSG_TAIL_SYM", sym)
         obj = HipSpectralStableColumnPreconditioner(A, protected=~lp.pos, precision=precision)
         obj._call("psg_set_weights", obj._ctx, theta, 1e-2)
         plan = SymbolicPlan(A, core)
         assert obj._call("psg_factor", obj._ctx, plan.handle, 0.0, None) == 0
-        info = np.zeros(6)
+        info = np.zeros(8)
         obj._call("psg_factor_info", obj._ctx, info)
         infos.append(info)
         r = np.random.default_rng(74).standard_normal(m)
@@ -292,10 +295,11 @@ def test_dense_tail_inverse_solve(gpu, monkeypatch, precision, m, n, k, frac, ta
         obj.close()
     tail_r = infos[1][3]
     assert infos[1][0] == 1 and tail_r >= tail_min and infos[1][4] >= 2
-    (l1, _), (s1, s2) = outs
+    assert infos[1][6] == 1 and infos[2][6] == 0 and infos[2][3] == tail_r
+    (l1, _), (s1, s2), (t1, t2) = outs
     tol = 1e-11 if precision == "fp64" else 1e-4
-    assert np.linalg.norm(s1 - l1) <= tol * np.linalg.norm(l1)
-    assert np.linalg.norm(s2 - s1) <= tol * np.linalg.norm(s1)
+    for a, b in ((s1, l1), (s2, s1), (t1, l1), (t2, t1)):
+        assert np.linalg.norm(a - b) <= tol * np.linalg.norm(b)
     apply_ref, cond = _dense_precond(A, core, 1.0 / theta, 1e-2)
     ref = apply_ref(r)
     assert np.linalg.norm(s1 - ref) <= (1e-9 if precision == "fp64" else 5e-6 * cond) * np.linalg.norm(ref)
