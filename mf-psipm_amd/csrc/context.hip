@@ -715,8 +715,10 @@ int psg_destroy(psg_ctx* c) {
   if (!c) return PSG_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   factor_destroy(c);
-  if (c->pcg_exec) (void)hipGraphExecDestroy(c->pcg_exec);
-  if (c->pcg_graph) (void)hipGraphDestroy(c->pcg_graph);
+  for (int g = 0; g < 2; ++g) {
+    if (c->pcg_exec[g]) (void)hipGraphExecDestroy(c->pcg_exec[g]);
+    if (c->pcg_graph[g]) (void)hipGraphDestroy(c->pcg_graph[g]);
+  }
   delete c->b2;
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -892,33 +894,48 @@ int psg_pcg(psg_ctx* c, double tol, int32_t maxit, int32_t replacement, double g
       HIP_LAUNCH_CHECK();
     };
     const int batch = std::max(1, c->pcg_batch);
-    // k iterations: graph replays of the captured iteration, or plain launches
+    // k iterations: graph replays, or plain launches.  A full batch is one
+    // graph holding `batch` iterations (no launch gap between them); a
+    // shorter run replays the one-iteration graph k times.
     auto run = [&](int k) {
       if (!c->pcg_use_graph) {
         for (int q = 0; q < k; ++q) iteration();
         return;
       }
       const void* bufs[3] = {c->rzs.p, c->alphas.p, c->betas.p};
-      if (!c->pcg_exec || c->pcg_gen != c->gen || !std::equal(bufs, bufs + 3, c->pcg_bufs)) {
-        if (c->pcg_exec) (void)hipGraphExecDestroy(c->pcg_exec);
-        if (c->pcg_graph) (void)hipGraphDestroy(c->pcg_graph);
-        c->pcg_exec = nullptr;
-        c->pcg_graph = nullptr;
-        HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        try {
-          iteration();
-        } catch (...) {
-          hipGraph_t g = nullptr;
-          (void)hipStreamEndCapture(s, &g);
-          if (g) (void)hipGraphDestroy(g);
-          throw;
+      if (c->pcg_gen != c->gen || !std::equal(bufs, bufs + 3, c->pcg_bufs)) {
+        for (int g = 0; g < 2; ++g) {
+          if (c->pcg_exec[g]) (void)hipGraphExecDestroy(c->pcg_exec[g]);
+          if (c->pcg_graph[g]) (void)hipGraphDestroy(c->pcg_graph[g]);
+          c->pcg_exec[g] = nullptr;
+          c->pcg_graph[g] = nullptr;
         }
-        HIP_CHECK(hipStreamEndCapture(s, &c->pcg_graph));
-        HIP_CHECK(hipGraphInstantiate(&c->pcg_exec, c->pcg_graph, nullptr, nullptr, 0));
         c->pcg_gen = c->gen;
         std::copy(bufs, bufs + 3, c->pcg_bufs);
       }
-      for (int q = 0; q < k; ++q) HIP_CHECK(hipGraphLaunch(c->pcg_exec, s));
+      const int g = (k == batch && batch > 1) ? 1 : 0;
+      const int per = g ? batch : 1;
+      if (c->pcg_exec[g] && c->pcg_graph_its[g] != per) {
+        (void)hipGraphExecDestroy(c->pcg_exec[g]);
+        (void)hipGraphDestroy(c->pcg_graph[g]);
+        c->pcg_exec[g] = nullptr;
+        c->pcg_graph[g] = nullptr;
+      }
+      if (!c->pcg_exec[g]) {
+        HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          for (int q = 0; q < per; ++q) iteration();
+        } catch (...) {
+          hipGraph_t gr = nullptr;
+          (void)hipStreamEndCapture(s, &gr);
+          if (gr) (void)hipGraphDestroy(gr);
+          throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(s, &c->pcg_graph[g]));
+        HIP_CHECK(hipGraphInstantiate(&c->pcg_exec[g], c->pcg_graph[g], nullptr, nullptr, 0));
+        c->pcg_graph_its[g] = per;
+      }
+      for (int q = 0; q < k; q += per) HIP_CHECK(hipGraphLaunch(c->pcg_exec[g], s));
     };
     while (info > 0 && it < maxit) {
       // iterations until the next replacement boundary or the limit: the
@@ -1537,7 +1554,42 @@ int psg_pcg2(psg_ctx* c, const double* tol, int32_t maxit, int32_t replacement, 
         if (replacement > 0) k = std::min(k, replacement - it[q] % replacement);
       }
       k = std::max(k, 1);
-      for (int j = 0; j < k; ++j) iteration();
+      if (!c->pcg_use_graph) {
+        for (int j = 0; j < k; ++j) iteration();
+      } else {
+        // as the single-column loop: a full batch is one graph of `batch`
+        // iterations, a shorter run replays the one-iteration graph
+        const void* bufs[5] = {b.rzs.p, b.alphas.p, b.betas.p, b.x.p, b.hvdots.p};
+        if (b.graph_gen != c->gen || b.graph_cap != cap || !std::equal(bufs, bufs + 5, b.graph_bufs)) {
+          b.drop_graphs();
+          b.graph_gen = c->gen;
+          b.graph_cap = cap;
+          std::copy(bufs, bufs + 5, b.graph_bufs);
+        }
+        const int g = (k == batch && batch > 1) ? 1 : 0;
+        const int per = g ? batch : 1;
+        if (b.exec[g] && b.graph_its[g] != per) {
+          (void)hipGraphExecDestroy(b.exec[g]);
+          (void)hipGraphDestroy(b.graph[g]);
+          b.exec[g] = nullptr;
+          b.graph[g] = nullptr;
+        }
+        if (!b.exec[g]) {
+          HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          try {
+            for (int j = 0; j < per; ++j) iteration();
+          } catch (...) {
+            hipGraph_t gr = nullptr;
+            (void)hipStreamEndCapture(s, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
+            throw;
+          }
+          HIP_CHECK(hipStreamEndCapture(s, &b.graph[g]));
+          HIP_CHECK(hipGraphInstantiate(&b.exec[g], b.graph[g], nullptr, nullptr, 0));
+          b.graph_its[g] = per;
+        }
+        for (int j = 0; j < k; j += per) HIP_CHECK(hipGraphLaunch(b.exec[g], s));
+      }
       HIP_CHECK(hipMemcpyAsync(b.hscal, b.scal.p, 2 * kScalarHeader * sizeof(double), hipMemcpyDeviceToHost, s));
       HIP_CHECK(hipStreamSynchronize(s));
       for (int q = 0; q < 2; ++q) {

@@ -37,7 +37,25 @@ struct Block2 {
   DevBuf<double> partA, partC, partD, hvdots, scal, rzs, alphas, betas;
   double* hscal = nullptr;                // pinned mirror of scal
   bool have_rhs = false, have_solution = false;
+  // captured pair iterations ([0] one, [1] a batch), keyed like the
+  // single-column graphs by the context's generation and the buffers
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  int graph_its[2] = {0, 0};
+  uint64_t graph_gen = 0;
+  const void* graph_bufs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int graph_cap = 0;
+  void drop_graphs() {
+    for (int g = 0; g < 2; ++g) {
+      if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
+      if (graph[g]) (void)hipGraphDestroy(graph[g]);
+      exec[g] = nullptr;
+      graph[g] = nullptr;
+      graph_its[g] = 0;
+    }
+  }
   ~Block2() {
+    drop_graphs();
     if (hscal) (void)hipHostFree(hscal);
   }
 };
@@ -61,8 +79,10 @@ struct psg_ctx {
   // state, a solve error), and the graph is captured again
   int pcg_use_graph = 1;
   uint64_t gen = 1, pcg_gen = 0;
-  hipGraph_t pcg_graph = nullptr;
-  hipGraphExec_t pcg_exec = nullptr;
+  // captured single-column PCG iterations: [0] one iteration, [1] a batch
+  hipGraph_t pcg_graph[2] = {nullptr, nullptr};
+  hipGraphExec_t pcg_exec[2] = {nullptr, nullptr};
+  int pcg_graph_its[2] = {0, 0};
   const void* pcg_bufs[3] = {nullptr, nullptr, nullptr};
   int nsplit = 1;             // column blocks of the CSR copy (rptr holds nsplit x (m+1))
   // Newton matrix

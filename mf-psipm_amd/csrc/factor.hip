@@ -413,6 +413,38 @@ __global__ void k_perm_in(int m, const int* __restrict__ perm, const double* __r
   }
 }
 
+// b = P D r as k_perm_in, and the isolated 1x1 supernodes (rows no active
+// column touches) solved in the same launch from their own entries:
+// y_k = l_kk^-1 b_k, x_k = l_kk^-1 y_k
+template <class T, int NR = 1>
+__global__ void k_perm_in_iso(int m, const int* __restrict__ perm, const double* __restrict__ dsc,
+                              const double* __restrict__ r, T* __restrict__ b, int n_iso,
+                              const int* __restrict__ iso_col, const int* __restrict__ iso_doff,
+                              const T* __restrict__ dinv, T* __restrict__ y, T* __restrict__ x) {
+  const int64_t nmax = max((int64_t)m, (int64_t)n_iso);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nmax; k += (int64_t)gridDim.x * blockDim.x) {
+    if (k < m) {
+      const int i = perm[k];
+      const double d = dsc[i];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) b[k * NR + q] = (T)(d * r[(int64_t)i * NR + q]);
+    }
+    if (k < n_iso) {
+      const int col = iso_col[k];
+      const int i = perm[col];
+      const double d = dsc[i];
+      const T di = dinv[iso_doff[k]];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        const T bk = (T)(d * r[(int64_t)i * NR + q]);
+        const T yk = di * bk;
+        y[(int64_t)col * NR + q] = yk;
+        x[(int64_t)col * NR + q] = di * yk;
+      }
+    }
+  }
+}
+
 template <class T, int NR = 1>
 __global__ void k_perm_out(int m, const int* __restrict__ perm, const double* __restrict__ dsc,
                            const T* __restrict__ x, double* __restrict__ z) {
@@ -1491,40 +1523,58 @@ k_tail_symv(int r, int ld, int sc, const int* __restrict__ tiles, const T* __res
     }
   __syncthreads();
   const int jend = min(j0 + sc, r);
-  for (int j = j0; j < jend; ++j) {
-    const T* col = Q + (int64_t)j * ld;
-    double bj[NR], cacc[NR];
+  // two columns per step: both columns' loads are in flight together, and
+  // one exchange across the wave halves starts both column reductions
+  for (int j = j0; j < jend; j += 2) {
+    const bool two = j + 1 < jend;  // uniform
+    VT v[2][KV];
 #pragma unroll
-    for (int e = 0; e < NR; ++e) {
-      bj[e] = bj_s[e][j - j0];
-      cacc[e] = 0.0;
+    for (int h = 0; h < 2; ++h) {
+      const int jj = j + h;
+      const T* col = Q + (int64_t)jj * ld;
+#pragma unroll
+      for (int k = 0; k < KV; ++k) {
+        const int i = iw + k * kWave * V;
+        if ((h == 0 || two) && i + V > jj && i < r) v[h][k] = *reinterpret_cast<const VT*>(col + i);
+      }
     }
-    VT v[KV];
+    double cacc[2][NR];
 #pragma unroll
-    for (int k = 0; k < KV; ++k) {
-      const int i = iw + k * kWave * V;
-      if (i + V > j && i < r) v[k] = *reinterpret_cast<const VT*>(col + i);
-    }
+    for (int h = 0; h < 2; ++h) {
 #pragma unroll
-    for (int k = 0; k < KV; ++k) {
-      const int i = iw + k * kWave * V;
-      if (!(i + V > j && i < r)) continue;
+      for (int e = 0; e < NR; ++e) cacc[h][e] = 0.0;
+      if (h == 1 && !two) continue;
+      const int jj = j + h;
+      double bj[NR];
 #pragma unroll
-      for (int t = 0; t < V; ++t) {
-        const int row = i + t;
-        const double q = (row >= j && row < r) ? (double)v[k][t] : 0.0;
-        const double qo = row > j ? q : 0.0;
+      for (int e = 0; e < NR; ++e) bj[e] = bj_s[e][jj - j0];
 #pragma unroll
-        for (int e = 0; e < NR; ++e) {
-          racc[k][t][e] += q * bj[e];
-          cacc[e] += qo * bi[k][t][e];
+      for (int k = 0; k < KV; ++k) {
+        const int i = iw + k * kWave * V;
+        if (!(i + V > jj && i < r)) continue;
+#pragma unroll
+        for (int t = 0; t < V; ++t) {
+          const int row = i + t;
+          const double q = (row >= jj && row < r) ? (double)v[h][k][t] : 0.0;
+          const double qo = row > jj ? q : 0.0;
+#pragma unroll
+          for (int e = 0; e < NR; ++e) {
+            racc[k][t][e] += q * bj[e];
+            cacc[h][e] += qo * bi[k][t][e];
+          }
         }
       }
     }
+    // lanes < 32 keep column j and take the upper half's column-j partial,
+    // lanes >= 32 the reverse; lane 0 ends with column j, lane 32 with j + 1
+    const bool hi = lane & 32;
 #pragma unroll
     for (int e = 0; e < NR; ++e) {
-      const double sm = wave_sum(cacc[e]);
+      double sm = (hi ? cacc[1][e] : cacc[0][e]) + __shfl_xor(hi ? cacc[0][e] : cacc[1][e], 32, kWave);
+#pragma unroll
+      for (int o = 16; o > 0; o >>= 1) sm += __shfl_xor(sm, o, kWave);
       if (lane == 0) cpart[wv][e][j - j0] = sm;
+      if (lane == 32 && two) cpart[wv][e][j + 1 - j0] = sm;
     }
   }
   __syncthreads();
@@ -1593,8 +1643,8 @@ static void setup_tail(psg_ctx* c, const Plan& P) {
   fs->tail_lower_tiles.upload(lt, st);
   fs->tail_upper_tiles.upload(ut, st);
   setup_dense_schedule(fs, r, c->precision, st);
-  // symmetric product: the widest tiles that still give ~1000 workgroups
-  // (columns sc in {16, 32, 64}, vectors per lane kv in {1, 2, 4})
+  // symmetric product: tiles of sc columns (64, halved down to 16 while
+  // fewer than ~1000 workgroups) x kBlock V kv rows
   // The explicit Q = X' X keeps its positive definiteness only while
   // cond(L_T L_T') stays well below 1 / (unit roundoff): always for FP64 at
   // the shifts the ladder uses, but not for FP32, where x = X' (X b) keeps
@@ -1604,7 +1654,11 @@ static void setup_tail(psg_ctx* c, const Plan& P) {
   if (fs->tail_sym) {
     const int V = c->precision == PSG_MIXED ? Vec16<float>::n : Vec16<double>::n;
     fs->tail_ld = (r + 3) / 4 * 4;
-    int sc = kSC, kv = V == 4 ? 2 : 4;  // 2048-row tiles (FP32: register budget)
+    // one vector per lane (512-row FP64 tiles) measured fastest: more
+    // resident waves beat more loads per wave; PSG_TAIL_KV = 2 or 4 widens
+    int sc = kSC, kv = 1;
+    const char* kvs = getenv("PSG_TAIL_KV");
+    if (kvs) kv = atoi(kvs) >= 4 ? 4 : (atoi(kvs) >= 2 ? 2 : 1);
     auto ntiles = [&](int sc_, int kv_) {
       const int rt = kBlock * V * kv_;
       int64_t n = 0;
@@ -2250,22 +2304,6 @@ static void apply_run(psg_ctx* c, const double* r, double* z, const T* panels, c
 }
 
 
-// isolated 1x1 supernodes: x_k = dinv_k^2 b_k (forward and backward at once)
-template <class T, int NR = 1>
-__global__ void k_iso_solve(int n, const int* __restrict__ col, const int* __restrict__ doff,
-                            const T* __restrict__ dinv, const T* __restrict__ b, T* __restrict__ y,
-                            T* __restrict__ x) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int k = col[i];
-    const T d = dinv[doff[i]];
-#pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      const T yk = d * b[(int64_t)k * NR + q];
-      y[(int64_t)k * NR + q] = yk;
-      x[(int64_t)k * NR + q] = d * yk;
-    }
-  }
-}
 
 template <class T, int NR = 1>
 static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels, const T* dinv, T* vb, T* vy,
@@ -2278,8 +2316,8 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   // solve's last kernel; cleared here only after a new factor or a failure
   const bool clean = fs->sf_clean == NR;
   if (!clean) zero_async(fs->flags.p, (int64_t)fs->flags.n, s);
-  hipLaunchKernelGGL((k_perm_in<T, NR>), dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, s, m, fs->perm.p, c->dsc.p,
-                     r, vb);
+  hipLaunchKernelGGL((k_perm_in_iso<T, NR>), dim3(grid_for(std::max(m, fs->n_iso), kBlock, 4096)), dim3(kBlock), 0, s,
+                     m, fs->perm.p, c->dsc.p, r, vb, fs->n_iso, fs->iso_col.p, fs->iso_doff.p, dinv, vy, vx);
   HIP_LAUNCH_CHECK();
   SolveArgs<T> a;
   a.nunits = fs->sf_units;
@@ -2313,11 +2351,6 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   a.trace = fs->trace.p;
   if (!clean) zero_async(a.acc, (int64_t)m * NR, s);
   const int grid = std::max(1, std::min(fs->sf_units, fs->sf_grid));
-  if (fs->n_iso) {
-    hipLaunchKernelGGL((k_iso_solve<T, NR>), dim3(grid_for(fs->n_iso, kBlock, 4096)), dim3(kBlock), 0, s, fs->n_iso,
-                       fs->iso_col.p, fs->iso_doff.p, dinv, vb, vy, vx);
-    HIP_LAUNCH_CHECK();
-  }
   const int nlow = (int)fs->low_ptr.size() - 1;
   const int wpb = kBlock / kWave;
   for (int L = 0; L < nlow; ++L) {
