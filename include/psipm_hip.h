@@ -292,6 +292,33 @@ int psg_select_features(psg_ctx* ctx, const int32_t* cand, int64_t ncand, const 
 int psg_select_logits(psg_ctx* ctx, psg_mlp* net, const int32_t* cand, int64_t ncand, const double* x,
                       const double* z, double mu, double iter, float* logits);
 
+/* ---- network tree preconditioner --------------------------------------------
+ * Replaces psipm_v3_hsd.py:653-701 (TreePreconditioner.factor / solve): the
+ * diagonal of A_C W A_C' + delta I plus its entries on the edges of a
+ * spanning forest (psipm_v3_hsd.py:436-557, built on the host), factored
+ * exactly (no fill) and solved by pointer jumping on the device.  After
+ * psg_tree_factor the context's PCG (psg_pcg, psg_pcg2, psg_apply_P) applies
+ * this preconditioner; psg_factor switches back to the Cholesky factor.
+ *
+ * psg_tree_setup: the forest of the context's m rows.  parent[m] (-1 at a
+ * root), order[m] = rows sorted by depth (ascending index within a depth),
+ * level_start[max_depth + 2] = offsets of the depths in order.  For rounds
+ * = ceil(log2(max_depth + 1)) pointer-jumping rounds k: jump[k m + i] = the
+ * ancestor 2^k levels above i (-1 past the root; jump[0 .. m) = parent), and
+ * the rows y with jump_k(y) = x listed in ascending order at child_idx
+ * [child_ptr[k (m + 1) + x], child_ptr[k (m + 1) + x + 1]) (absolute
+ * offsets into child_idx).  arc_ptr[m + 1] / arc_idx / arc_coef: per row x,
+ * the columns of A parallel to x's tree edge and a_x,j a_parent,j. */
+int psg_tree_setup(psg_ctx* ctx, int32_t max_depth, const int32_t* parent, const int32_t* order,
+                   const int64_t* level_start, int32_t rounds, const int32_t* jump, const int64_t* child_ptr,
+                   const int32_t* child_idx, const int64_t* arc_ptr, const int32_t* arc_idx, const double* arc_coef);
+/* factor with the context's weights and delta (psg_set_weights) over the
+ * active columns (n bytes, nonzero = active; NULL = all columns) */
+int psg_tree_factor(psg_ctx* ctx, const uint8_t* active);
+/* pivots (after the clip at max(delta 1e-8, tiny)) and multipliers, m each
+ * (either nullable) */
+int psg_tree_export(psg_ctx* ctx, double* pivot, double* mult);
+
 /* device time (ms, HIP events) of the last psg_pcg and last psg_factor calls */
 int psg_last_timings(psg_ctx* ctx, double* pcg_ms, double* factor_ms);
 

@@ -91,6 +91,10 @@ SIGNATURES = [
     ("psg_last_timings", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("psg_factor_info", C.c_int, [C.c_void_p, _f64p]),
     ("psg_dense_cholinv", C.c_int, [C.c_void_p, C.c_int64, _f64p]),
+    ("psg_tree_setup", C.c_int, [C.c_void_p, C.c_int32, _i32p, _i32p, _i64p, C.c_int32, _opt(_i32p), _opt(_i64p),
+                                 _opt(_i32p), _i64p, _opt(_i32p), _opt(_f64p)]),
+    ("psg_tree_factor", C.c_int, [C.c_void_p, _opt(_u8p)]),
+    ("psg_tree_export", C.c_int, [C.c_void_p, _opt(_f64p), _opt(_f64p)]),
     ("psg_solve_begin2", C.c_int, [C.c_void_p, _f64p, _f64p]),
     ("psg_pcg2", C.c_int, [C.c_void_p, _f64p, C.c_int32, C.c_int32, C.c_double, C.POINTER(PsgPcgResult),
                            _opt(_f64p), _opt(_f64p), _opt(_f64p)]),

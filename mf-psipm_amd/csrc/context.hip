@@ -715,6 +715,7 @@ int psg_destroy(psg_ctx* c) {
   if (!c) return PSG_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   factor_destroy(c);
+  tree_destroy(c);
   for (int g = 0; g < 2; ++g) {
     if (c->pcg_exec[g]) (void)hipGraphExecDestroy(c->pcg_exec[g]);
     if (c->pcg_graph[g]) (void)hipGraphDestroy(c->pcg_graph[g]);

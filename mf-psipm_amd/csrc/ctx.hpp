@@ -6,6 +6,7 @@
 namespace psg {
 
 struct FactorState;  // factor.hip
+struct TreeState;    // tree.hip
 
 // layout of the small host-visible scalar block (doubles)
 enum ScalarSlot {
@@ -114,6 +115,11 @@ struct psg_ctx {
   double last_pcg_ms = 0.0, last_factor_ms = 0.0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   psg::FactorState* fs = nullptr;
+  // preconditioner the PCG applies: 0 the supernodal Cholesky factor
+  // (psg_factor selects it), 1 the tree-plus-diagonal factor of a network
+  // (psg_tree_factor selects it)
+  int precond = 0;
+  psg::TreeState* tree = nullptr;
   psg::Block2* b2 = nullptr;
   // interior-point iterate for the step-length kernels (psg_ipm_observe)
   psg::DevBuf<double> ipm_x, ipm_z, ipm_s, ipm_w, ipm_d1, ipm_d2;
@@ -145,6 +151,10 @@ void factor_apply2(psg_ctx* ctx, const double* r, double* z);
 int* factor_knob(psg_ctx* ctx, int knob);
 // nonzero if a dependency wait of the last single-launch solve timed out
 int factor_solve_error(psg_ctx* ctx);
+// tree.hip: the network tree preconditioner
+bool tree_ready(const psg_ctx* ctx);
+void tree_apply(psg_ctx* ctx, const double* r, double* z, int nr);
+void tree_destroy(psg_ctx* ctx);
 // context.hip helpers used by factor.hip
 // factor_only: leave the Woodbury columns out of d_full (the factor's diagonal)
 void diag_terms_device(psg_ctx* ctx, const uint8_t* mask_dev, double* full, double* core, bool factor_only);

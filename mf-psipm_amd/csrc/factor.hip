@@ -2404,7 +2404,7 @@ int* factor_knob(psg_ctx* c, int knob) {
 
 int factor_solve_error(psg_ctx* c) {
   FactorState* fs = c->fs;
-  if (!fs || fs->solve_mode == 0 || !fs->flags.p) return 0;
+  if (c->precond == 1 || !fs || fs->solve_mode == 0 || !fs->flags.p) return 0;  // the tree solve has no waits
   int e = 0;
   HIP_CHECK(hipMemcpyAsync(&e, fs->flags.p + 4 * fs->nsuper + 4, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -2429,7 +2429,10 @@ void factor_destroy(psg_ctx* c) {
   c->fs = nullptr;
 }
 
-bool factor_ready(const psg_ctx* c) { return c->fs && c->fs->ready; }
+bool factor_ready(const psg_ctx* c) {
+  if (c->precond == 1) return tree_ready(c);
+  return c->fs && c->fs->ready;
+}
 
 int factor_numeric(psg_ctx* c, const Plan& P, double eta, double* nnz_factor) {
   if (!c->fs) c->fs = new FactorState;
@@ -2437,6 +2440,7 @@ int factor_numeric(psg_ctx* c, const Plan& P, double eta, double* nnz_factor) {
   if (c->fs->serial != P.serial || c->fs->precision != c->precision) upload_plan(c, P);
   c->fs->ready = false;
   c->fs->sf_clean = 0;
+  c->precond = 0;
   c->gen++;  // the factor's layout and buffers are captured PCG launch arguments
   HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   int status;
@@ -2455,6 +2459,10 @@ int factor_numeric(psg_ctx* c, const Plan& P, double eta, double* nnz_factor) {
 }
 
 void factor_apply(psg_ctx* c, const double* r, double* z) {
+  if (c->precond == 1) {
+    tree_apply(c, r, z, 1);
+    return;
+  }
   FactorState* fs = c->fs;
   if (fs->solve_mode == 1) {
     if (c->precision == PSG_MIXED)
@@ -2472,6 +2480,10 @@ void factor_apply(psg_ctx* c, const double* r, double* z) {
 // Two right-hand sides at once (vectors interleaved, entry (i, q) at 2 i + q):
 // every panel entry and block inverse is read once for both.
 void factor_apply2(psg_ctx* c, const double* r, double* z) {
+  if (c->precond == 1) {
+    tree_apply(c, r, z, 2);
+    return;
+  }
   FactorState* fs = c->fs;
   if (fs->solve_mode != 1) throw Error(PSG_ERR_STATE, "two-column solves need the single-launch solve layout");
   if (c->precision == PSG_MIXED)

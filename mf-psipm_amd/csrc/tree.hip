@@ -1,0 +1,346 @@
+// Tree-plus-diagonal preconditioner of network (incidence-matrix) LPs.
+//
+// Replaces, on the GPU, psipm_v3_hsd.py:653-701 (TreePreconditioner): the
+// matrix keeps the diagonal of A_C W A_C' + delta I and, off the diagonal,
+// only the entries on the edges of a spanning forest of the network (built
+// on the host, psipm_v3_hsd.py:436-557).  A tree has no fill, so its LDL'
+// factor is exact:
+//   * factor (psipm_v3_hsd.py:667-690): from the deepest tree level up,
+//     pivot_x = diag_x - sum over children c of edge_c^2 / max(pivot_c, tiny)
+//     and mult_x = -edge_x / max(pivot_x, tiny); one workgroup walks the
+//     levels (a level only needs the one below it), children in ascending
+//     index order, so the pivots equal the reference's level loop bit for
+//     bit; pivots are clipped at max(delta 1e-8, tiny) afterwards;
+//   * solve (psipm_v3_hsd.py:692-701): the forward sweep u_p = v_p + sum_c
+//     g_c u_c (g = -mult), the division by the pivots, and the backward sweep
+//     x_c = v_c + g_c x_parent.  On a 1000 x 1000 grid the trees are ~1000
+//     levels deep, so instead of one launch per level the sweeps run by
+//     pointer jumping in ceil(log2(depth + 1)) rounds each: with J_k the
+//     ancestor 2^k levels up and W_k the product of g along that path,
+//       forward  s_{k+1}(x) = s_k(x) + sum_{y : J_k(y) = x} W_k(y) s_k(y)
+//       backward t_{k+1}(y) = t_k(y) + W_k(y) t_k(J_k(y))
+//     (every descendant, resp. ancestor, at distance d is reached once,
+//     through the binary digits of d).  The forward rounds gather over
+//     host-built lists of the 2^k-descendants, so both directions are
+//     deterministic gathers with no atomics.
+#include "ctx.hpp"
+
+namespace psg {
+
+constexpr double kTiny = 2.2250738585072014e-308;  // smallest normal double
+
+struct TreeState {
+  int m = 0, n = 0, rounds = 0, max_depth = 0;
+  bool ready = false;
+  std::vector<int64_t> level_start;  // host copy: max_depth + 2 offsets into order
+  DevBuf<int> parent, order, jump, cidx, arc_idx;
+  DevBuf<int64_t> dstart, cptr, arc_ptr;
+  DevBuf<double> arc_coef, terms, edge, pivot, pclip, mult, wk, s0, s1;
+  DevBuf<uint8_t> mask;
+};
+
+// edge_x = -sum over the active arcs parallel to x's tree edge of
+// a_x,j a_parent,j w_j (+sum w_j for +-1 incidence)
+__global__ void k_tree_edge(int m, const int64_t* __restrict__ ptr, const int* __restrict__ arc,
+                            const double* __restrict__ coef, const double* __restrict__ w,
+                            const uint8_t* __restrict__ mask, double* __restrict__ edge) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (int64_t)gridDim.x * blockDim.x) {
+    double e = 0.0;
+    for (int64_t k = ptr[x]; k < ptr[x + 1]; ++k) {
+      const int j = arc[k];
+      if (!mask || mask[j]) e -= coef[k] * w[j];
+    }
+    edge[x] = e;
+  }
+}
+
+// One workgroup: the levels from the deepest up; the children of a level-L
+// node sit at level L + 1, already final after the previous barrier.
+__global__ void __launch_bounds__(1024)
+k_tree_factor(int max_depth, const int64_t* __restrict__ dstart, const int* __restrict__ order,
+              const int64_t* __restrict__ cptr, const int* __restrict__ cidx, const double* __restrict__ terms,
+              double delta, const double* __restrict__ edge, double* __restrict__ pivot, double* __restrict__ pclip,
+              double* __restrict__ mult) {
+  for (int L = max_depth; L >= 0; --L) {
+    for (int64_t q = dstart[L] + threadIdx.x; q < dstart[L + 1]; q += blockDim.x) {
+      const int x = order[q];
+      double p = delta + terms[x];
+      for (int64_t k = cptr[x]; k < cptr[x + 1]; ++k) {
+        const int c = cidx[k];
+        const double e = edge[c];
+        p -= (e * e) / pclip[c];
+      }
+      pivot[x] = p;
+      const double pc = fmax(p, kTiny);
+      pclip[x] = pc;
+      mult[x] = L > 0 ? -edge[x] / pc : 0.0;
+    }
+    __syncthreads();
+  }
+}
+
+// pivots clipped for the solve; W_0 = g = -mult on the tree edges
+__global__ void k_tree_w0(int m, const int* __restrict__ parent, const double* __restrict__ mult, double floor_,
+                          double* __restrict__ pivot, double* __restrict__ w0) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (int64_t)gridDim.x * blockDim.x) {
+    pivot[x] = fmax(pivot[x], floor_);
+    w0[x] = parent[x] >= 0 ? -mult[x] : 0.0;
+  }
+}
+
+// W_k(y) = W_{k-1}(y) W_{k-1}(J_{k-1}(y)): the path y -> J_{k-1}(y) -> J_k(y)
+__global__ void k_tree_wk(int m, const int* __restrict__ jprev, const double* __restrict__ wprev,
+                          double* __restrict__ wk) {
+  for (int64_t y = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; y < m; y += (int64_t)gridDim.x * blockDim.x) {
+    const int a = jprev[y];
+    wk[y] = a >= 0 ? wprev[y] * wprev[a] : 0.0;
+  }
+}
+
+// forward round k: out(x) = in(x) + sum over the 2^k-descendants y of
+// W_k(y) in(y); the last round divides by the pivots
+template <int NR>
+__global__ void k_tree_fwd(int m, const int64_t* __restrict__ ptr, const int* __restrict__ idx,
+                           const double* __restrict__ wk, const double* __restrict__ pivot, bool last,
+                           const double* __restrict__ in, double* __restrict__ out) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (int64_t)gridDim.x * blockDim.x) {
+    double s[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) s[q] = in[x * NR + q];
+    for (int64_t k = ptr[x]; k < ptr[x + 1]; ++k) {
+      const int y = idx[k];
+      const double g = wk[y];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) s[q] += g * in[(int64_t)y * NR + q];
+    }
+    if (last) {
+      const double p = pivot[x];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) s[q] /= p;
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) out[x * NR + q] = s[q];
+  }
+}
+
+// backward round k: out(y) = in(y) + W_k(y) in(J_k(y))
+template <int NR>
+__global__ void k_tree_bwd(int m, const int* __restrict__ jump, const double* __restrict__ wk,
+                           const double* __restrict__ in, double* __restrict__ out) {
+  for (int64_t y = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; y < m; y += (int64_t)gridDim.x * blockDim.x) {
+    const int a = jump[y];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      double t = in[y * NR + q];
+      if (a >= 0) t += wk[y] * in[(int64_t)a * NR + q];
+      out[y * NR + q] = t;
+    }
+  }
+}
+
+// no tree edges at all: x = r / pivot
+template <int NR>
+__global__ void k_tree_diag_only(int m, const double* __restrict__ pivot, const double* __restrict__ in,
+                                 double* __restrict__ out) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) out[x * NR + q] = in[x * NR + q] / pivot[x];
+  }
+}
+
+template <int NR>
+static void tree_apply_nr(psg_ctx* c, const double* r, double* z) {
+  TreeState* t = c->tree;
+  hipStream_t s = c->stream;
+  const int m = t->m;
+  const int grid = grid_for(m, kBlock, 4096);
+  if (t->rounds == 0) {
+    hipLaunchKernelGGL((k_tree_diag_only<NR>), dim3(grid), dim3(kBlock), 0, s, m, t->pivot.p, r, z);
+    HIP_LAUNCH_CHECK();
+    return;
+  }
+  // forward rounds r -> s0/s1 ..., backward rounds ... -> z (the last one)
+  double* buf[2] = {t->s0.p, t->s1.p};
+  const double* in = r;
+  int next = 0;
+  for (int k = 0; k < t->rounds; ++k) {
+    double* out = buf[next];
+    hipLaunchKernelGGL((k_tree_fwd<NR>), dim3(grid), dim3(kBlock), 0, s, m, t->cptr.p + (int64_t)k * (m + 1), t->cidx.p,
+                       t->wk.p + (int64_t)k * m, t->pivot.p, k == t->rounds - 1, in, out);
+    HIP_LAUNCH_CHECK();
+    in = out;
+    next ^= 1;
+  }
+  for (int k = 0; k < t->rounds; ++k) {
+    double* out = (k == t->rounds - 1) ? z : buf[next];
+    hipLaunchKernelGGL((k_tree_bwd<NR>), dim3(grid), dim3(kBlock), 0, s, m, t->jump.p + (int64_t)k * m,
+                       t->wk.p + (int64_t)k * m, in, out);
+    HIP_LAUNCH_CHECK();
+    in = out;
+    next ^= 1;
+  }
+}
+
+void tree_apply(psg_ctx* c, const double* r, double* z, int nr) {
+  if (nr == 2)
+    tree_apply_nr<2>(c, r, z);
+  else
+    tree_apply_nr<1>(c, r, z);
+}
+
+bool tree_ready(const psg_ctx* c) { return c->tree && c->tree->ready; }
+
+void tree_destroy(psg_ctx* c) {
+  delete c->tree;
+  c->tree = nullptr;
+}
+
+}  // namespace psg
+
+using namespace psg;
+
+extern "C" {
+
+int psg_tree_setup(psg_ctx* c, int32_t max_depth, const int32_t* parent, const int32_t* order,
+                   const int64_t* level_start, int32_t rounds, const int32_t* jump, const int64_t* child_ptr,
+                   const int32_t* child_idx, const int64_t* arc_ptr, const int32_t* arc_idx, const double* arc_coef) {
+  return guarded([&]() -> int {
+    if (!c || !parent || !order || !level_start || !arc_ptr || max_depth < 0 || rounds < 0 || rounds > 31)
+      throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: invalid arguments");
+    if (rounds > 0 && (!jump || !child_ptr)) throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: missing jump tables");
+    if ((int64_t)1 << rounds <= max_depth) throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: too few rounds for the depth");
+    const int m = c->m, n = c->n;
+    // host-side validation: every index the kernels follow stays in range
+    std::vector<char> seen(m, 0);
+    for (int i = 0; i < m; ++i) {
+      if (parent[i] < -1 || parent[i] >= m) throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: parent out of range");
+      const int x = order[i];
+      if (x < 0 || x >= m || seen[x]) throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: order is not a permutation");
+      seen[x] = 1;
+    }
+    if (level_start[0] != 0 || level_start[max_depth + 1] != m)
+      throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: level offsets do not span the rows");
+    for (int L = 0; L <= max_depth; ++L)
+      if (level_start[L + 1] < level_start[L]) throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: level offsets decrease");
+    int64_t nc = 0;
+    if (rounds > 0) {
+      nc = child_ptr[(int64_t)(rounds - 1) * (m + 1) + m];
+      if (nc > 0 && !child_idx) throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: missing child lists");
+      for (int k = 0; k < rounds; ++k) {
+        const int64_t* p = child_ptr + (int64_t)k * (m + 1);
+        for (int i = 0; i < m; ++i)
+          if (p[i + 1] < p[i] || p[i] < 0 || p[i + 1] > nc)
+            throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: child offsets out of range");
+        for (int i = 0; i < m; ++i) {
+          const int a = jump[(int64_t)k * m + i];
+          if (a < -1 || a >= m) throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: jump out of range");
+        }
+      }
+      for (int64_t k = 0; k < nc; ++k)
+        if (child_idx[k] < 0 || child_idx[k] >= m) throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: child out of range");
+    }
+    const int64_t na = arc_ptr[m];
+    if (na < 0 || (na > 0 && (!arc_idx || !arc_coef))) throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: missing arc lists");
+    for (int i = 0; i < m; ++i)
+      if (arc_ptr[i + 1] < arc_ptr[i] || arc_ptr[i] < 0) throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: arc offsets");
+    for (int64_t k = 0; k < na; ++k)
+      if (arc_idx[k] < 0 || arc_idx[k] >= n) throw Error(PSG_ERR_ARGUMENT, "psg_tree_setup: arc out of range");
+    if (!c->tree) c->tree = new TreeState;
+    TreeState* t = c->tree;
+    hipStream_t s = c->stream;
+    t->ready = false;
+    t->m = m;
+    t->n = n;
+    t->rounds = rounds;
+    t->max_depth = max_depth;
+    t->level_start.assign(level_start, level_start + max_depth + 2);
+    t->parent.upload(parent, m, s);
+    t->order.upload(order, m, s);
+    t->dstart.upload(level_start, max_depth + 2, s);
+    // the factor walks the children of round 0; a depth-0 forest still
+    // needs empty lists
+    std::vector<int64_t> zero_ptr;
+    if (rounds > 0) {
+      t->jump.upload(jump, (size_t)rounds * m, s);
+      t->cptr.upload(child_ptr, (size_t)rounds * (m + 1), s);
+      t->cidx.upload(child_idx, std::max<int64_t>(nc, 1), s);
+    } else {
+      zero_ptr.assign(m + 1, 0);
+      t->cptr.upload(zero_ptr, s);
+      t->cidx.alloc(1);
+    }
+    t->arc_ptr.upload(arc_ptr, m + 1, s);
+    t->arc_idx.alloc(std::max<int64_t>(na, 1));
+    t->arc_coef.alloc(std::max<int64_t>(na, 1));
+    if (na) {
+      t->arc_idx.upload(arc_idx, na, s);
+      t->arc_coef.upload(arc_coef, na, s);
+    }
+    for (DevBuf<double>* b : {&t->terms, &t->edge, &t->pivot, &t->pclip, &t->mult}) b->alloc(m);
+    t->wk.alloc((size_t)std::max(rounds, 1) * m);
+    t->s0.alloc((size_t)m * kMaxRhs);
+    t->s1.alloc((size_t)m * kMaxRhs);
+    HIP_CHECK(hipStreamSynchronize(s));
+    return PSG_OK;
+  });
+}
+
+int psg_tree_factor(psg_ctx* c, const uint8_t* active) {
+  return guarded([&]() -> int {
+    if (!c) throw Error(PSG_ERR_ARGUMENT, "psg_tree_factor: null context");
+    if (!c->tree) throw Error(PSG_ERR_STATE, "psg_tree_factor: call psg_tree_setup first");
+    if (!c->have_weights) throw Error(PSG_ERR_STATE, "psg_tree_factor: weights not set (call psg_set_weights)");
+    TreeState* t = c->tree;
+    hipStream_t s = c->stream;
+    const int m = t->m;
+    const int grid = grid_for(m, kBlock, 4096);
+    HIP_CHECK(hipEventRecord(c->ev0, s));
+    const uint8_t* mask = nullptr;
+    if (active) {
+      t->mask.upload(active, t->n, s);
+      mask = t->mask.p;
+    }
+    // diagonal terms sum_j a_ij^2 w_j over the active arcs
+    if (mask)
+      diag_terms_device(c, mask, nullptr, t->terms.p, false);
+    else
+      diag_terms_device(c, nullptr, t->terms.p, nullptr, false);
+    hipLaunchKernelGGL(k_tree_edge, dim3(grid), dim3(kBlock), 0, s, m, t->arc_ptr.p, t->arc_idx.p, t->arc_coef.p,
+                       c->w.p, mask, t->edge.p);
+    HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tree_factor, dim3(1), dim3(1024), 0, s, t->max_depth, t->dstart.p, t->order.p, t->cptr.p,
+                       t->cidx.p, t->terms.p, c->delta, t->edge.p, t->pivot.p, t->pclip.p, t->mult.p);
+    HIP_LAUNCH_CHECK();
+    const double floor_ = std::max(c->delta * 1.0e-8, kTiny);
+    hipLaunchKernelGGL(k_tree_w0, dim3(grid), dim3(kBlock), 0, s, m, t->parent.p, t->mult.p, floor_, t->pivot.p,
+                       t->wk.p);
+    HIP_LAUNCH_CHECK();
+    for (int k = 1; k < t->rounds; ++k) {
+      hipLaunchKernelGGL(k_tree_wk, dim3(grid), dim3(kBlock), 0, s, m, t->jump.p + (int64_t)(k - 1) * m,
+                         t->wk.p + (int64_t)(k - 1) * m, t->wk.p + (int64_t)k * m);
+      HIP_LAUNCH_CHECK();
+    }
+    HIP_CHECK(hipEventRecord(c->ev1, s));
+    HIP_CHECK(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_factor_ms = ms;
+    t->ready = true;
+    c->precond = 1;
+    c->gen++;  // the PCG graphs capture the preconditioner's launches
+    return PSG_OK;
+  });
+}
+
+int psg_tree_export(psg_ctx* c, double* pivot, double* mult) {
+  return guarded([&]() -> int {
+    if (!c || !c->tree || !c->tree->ready) throw Error(PSG_ERR_STATE, "psg_tree_export: no tree factor");
+    TreeState* t = c->tree;
+    if (pivot) t->pivot.download(pivot, t->m, c->stream);
+    if (mult) t->mult.download(mult, t->m, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PSG_OK;
+  });
+}
+
+}  // extern "C"
