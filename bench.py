@@ -10,13 +10,17 @@ The preconditioner core is the reference's score-ranked active-column set
 capped by a fill budget (nnz(L) <= --fill-budget).
 
 A step is one PCG iteration of the plain FP64 GPU path: H p (two gathers),
-the fused step and dot kernels, P^-1 r (both supernodal triangular solves),
-and the direction update, with every input resident in HBM.  `value` is
-PCG iterations per second.  The line also reports the mixed-precision
-rate, the factorisation time, the HBM roofline of the H v kernels (the
-hot path's HBM-bound kernel pair) and a CPU baseline: this project's FP64
-CPU restatement (scipy SpMV + SuperLU factor of the same active-column
-matrix, single thread) timed on a bounded sample of PCG iterations.
+the fused step and dot kernels, P^-1 r (the supernodal triangular solves
+around the dense tail's symmetric product), and the direction update, with
+every input resident in HBM.  `value` is PCG iterations per second.  The
+line also reports the mixed-precision and two-column rates, the
+factorisation time, the HBM roofline of P^-1 r (the dominant kernel group)
+and of H v, and a CPU baseline: this project's FP64 CPU restatement (scipy
+SpMV + SuperLU factor of the same active-column matrix, single thread) timed
+on a bounded sample of PCG iterations.  Further legs (each can be switched
+off): time to a 1e-8 gap on a 2000 x 6000 random LP and a 2000 x 100,000
+set-covering LP, 64 random LPs solved as one batch, and the network tree
+preconditioner on a 1000 x 1000 grid min-cost flow.
 """
 from __future__ import annotations
 
@@ -195,6 +199,55 @@ def time_to_gap(m, n, k, seed, precision, pc=3, selector=None, problem=None, **o
     return out
 
 
+def network_leg(rows, steps):
+    """Tree-plus-diagonal preconditioner (V3:653-701) on the 1000 x 1000 grid
+    min-cost flow (grounded: 999,999 rows, 3,996,000 arcs): host setup,
+    device factor, P^-1 r and H v kernel times, PCG iterations/s with
+    theta = 10^U(-4,4)."""
+    import ctypes as C
+
+    import numpy as np
+
+    import mf_psipm_amd as P
+    from mf_psipm_amd import _lib
+    from mf_psipm_amd.network import HipTreePreconditioner
+
+    full = P.generators.grid_min_cost_flow(rows, rows, 500, 0)
+    lp, _ = P.lp.ground_network(full)
+    A = lp.A.tocsc()
+    m, n = A.shape
+    t0 = time.perf_counter()
+    obj = HipTreePreconditioner(A, cg_tol=1e-8, cg_tol_max=1e-8, cg_maxit=steps, residual_replacement=0)
+    setup = time.perf_counter() - t0
+    rng = np.random.default_rng(1)
+    theta = 10.0 ** rng.uniform(-4.0, 4.0, n)
+    obj.factorize(theta, 1e-6)
+    obj.factorize(theta, 1e-6)
+    fms = C.c_double()
+    obj._call("psg_last_timings", obj._ctx, None, C.byref(fms))
+    kt = np.zeros(4)
+    obj._call("psg_time_kernels", obj._ctx, 20, kt)
+    obj._call("psg_solve_begin", obj._ctx, np.zeros(n), rng.standard_normal(m))
+    res = _lib.PsgPcgResult()
+    obj._call("psg_pcg", obj._ctx, 1e-30, 5, 0, 0.1, C.byref(res), None, None, None)
+    t0 = time.perf_counter()
+    obj._call("psg_pcg", obj._ctx, 1e-30, steps, 0, 0.1, C.byref(res), None, None, None)
+    wall = time.perf_counter() - t0
+    rounds = obj.tree_rounds
+    # bytes one apply moves: per forward round the vector in/out and its
+    # gathered entries (8 B each way + 8 B gathered), the child offsets (8 B),
+    # child indices (4 B) and path weights (8 B); per backward round the
+    # jump pointers (4 B), path weights (8 B), the vector in/out and the
+    # gathered ancestor (8 B each)
+    apply_bytes = rounds * m * (8 + 8 + 8 + 8 + 4 + 8) + rounds * m * (4 + 8 + 8 + 8 + 8)
+    out = dict(rows=m, arcs=n, depth=obj.topology.max_depth, rounds=rounds, setup_s=round(setup, 3),
+               factor_ms=round(fms.value, 3), apply_ms=round(float(kt[2]), 4), hv_ms=round(float(kt[0]), 4),
+               pcg_it_per_s=round(res.iterations / wall, 1), apply_bytes=int(apply_bytes),
+               apply_gbs=round(float(apply_bytes / (kt[2] * 1e-3) / 1e9), 1))
+    obj.close()
+    return out
+
+
 def batch_leg(count, seed0, precision, core_factor, tol, maxit, graph_iters, sequential):
     """Independent random LPs (m ~ U{5000..50000}, n = 3m, 6 nnz/col) solved
     as one batch: one launch of each hot-path kernel per PCG step for all LPs,
@@ -267,6 +320,7 @@ def main():
     ap.add_argument("--no-pair", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-network", action="store_true")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-core-factor", type=float, default=0.1)
     ap.add_argument("--no-sequential", action="store_true")
@@ -314,6 +368,10 @@ def main():
             log(f"time-to-1e-8-gap set cover 2000x100000 {name}: {learned[name]}")
         learned["speedup"] = round(learned["plain_fp64"]["seconds"] / learned["learned_mixed_block"]["seconds"], 3)
         sel.close()
+    network = None
+    if not args.no_network:
+        network = network_leg(1000, 200)
+        log(f"network tree preconditioner 1000x1000 grid: {network}")
     batch = None
     if not args.no_batch:
         batch = batch_leg(args.batch_count, 100, "fp64", args.batch_core_factor, 1e-8, 1000, 8,
@@ -379,6 +437,7 @@ def main():
         "pair_column_it_per_s_mixed": None if mixed is None or mixed["pair"] is None
         else round(mixed["pair"]["it_per_s"], 3),
         "batch_64_lps": batch,
+        "network_tree_1000x1000": network,
         "autotune": None if g64["tuned"] is None else {k: v[0] for k, v in g64["tuned"].items()},
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["it_per_s"], 4), "unit": "PCG it/s", "cores": 1, "kind": "port",

@@ -29,6 +29,14 @@ namespace psg {
 
 constexpr double kTiny = 2.2250738585072014e-308;  // smallest normal double
 
+// A forward round gathers, for every row x, the rows 2^k levels below it.
+// Near the root of a grid's BFS tree that list is a whole ring (thousands
+// of rows) while most lists hold one or two, so lists longer than kShort are
+// cut into chunks of at most kChunk entries, one wave each; the last wave of
+// a row (by an arrival counter) adds the chunk sums in chunk order.
+constexpr int kShort = 16;
+constexpr int kChunk = 256;
+
 struct TreeState {
   int m = 0, n = 0, rounds = 0, max_depth = 0;
   bool ready = false;
@@ -37,6 +45,12 @@ struct TreeState {
   DevBuf<int64_t> dstart, cptr, arc_ptr;
   DevBuf<double> arc_coef, terms, edge, pivot, pclip, mult, wk, s0, s1;
   DevBuf<uint8_t> mask;
+  // chunks of the long lists, round k at [chunk_off[k], chunk_off[k + 1])
+  std::vector<int64_t> chunk_off;
+  DevBuf<uint8_t> is_long;                    // rounds x m
+  DevBuf<int> ch_row, ch_first, ch_count, ch_slot, ch_cnt;
+  DevBuf<int64_t> ch_beg, ch_end;
+  DevBuf<double> ch_part;
 };
 
 // edge_x = -sum over the active arcs parallel to x's tree edge of
@@ -98,12 +112,62 @@ __global__ void k_tree_wk(int m, const int* __restrict__ jprev, const double* __
 }
 
 // forward round k: out(x) = in(x) + sum over the 2^k-descendants y of
-// W_k(y) in(y); the last round divides by the pivots
+// W_k(y) in(y); the last round divides by the pivots.  Blocks below
+// nchunk_blocks run the chunks of the long lists (one wave per chunk), the
+// others one short row per thread.
 template <int NR>
-__global__ void k_tree_fwd(int m, const int64_t* __restrict__ ptr, const int* __restrict__ idx,
-                           const double* __restrict__ wk, const double* __restrict__ pivot, bool last,
-                           const double* __restrict__ in, double* __restrict__ out) {
-  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < m; x += (int64_t)gridDim.x * blockDim.x) {
+__global__ void __launch_bounds__(kBlock)
+k_tree_fwd(int m, const int64_t* __restrict__ ptr, const int* __restrict__ idx, const uint8_t* __restrict__ is_long,
+           int nchunk_blocks, int nchunks, const int* __restrict__ ch_row, const int64_t* __restrict__ ch_beg,
+           const int64_t* __restrict__ ch_end, const int* __restrict__ ch_first, const int* __restrict__ ch_count,
+           const int* __restrict__ ch_slot, int* __restrict__ ch_cnt, double* __restrict__ part,
+           const double* __restrict__ wk, const double* __restrict__ pivot, bool last,
+           const double* __restrict__ in, double* __restrict__ out) {
+  if ((int)blockIdx.x < nchunk_blocks) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (c >= nchunks) return;  // uniform per wave
+    double s[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) s[q] = 0.0;
+    for (int64_t e = ch_beg[c] + lane; e < ch_end[c]; e += kWave) {
+      const int y = idx[e];
+      const double g = wk[y];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) s[q] += g * in[(int64_t)y * NR + q];
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) s[q] = wave_sum(s[q]);
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q)
+        __hip_atomic_store(part + (int64_t)c * NR + q, s[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_s_waitcnt(0);  // the chunk sum is performed before it is counted
+      const int old = __hip_atomic_fetch_add(ch_cnt + ch_slot[c], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (old == ch_count[c] - 1) {
+        // the row's last chunk: its sums in chunk order, read after the
+        // counter has returned
+        const int x = ch_row[c];
+        double t[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) t[q] = in[(int64_t)x * NR + q];
+        for (int k = 0; k < ch_count[c]; ++k) {
+          const int64_t o = (int64_t)(ch_first[c] + k) * NR;
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+            t[q] += __hip_atomic_load(part + o + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        const double p = last ? pivot[x] : 1.0;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) out[(int64_t)x * NR + q] = last ? t[q] / p : t[q];
+        __hip_atomic_store(ch_cnt + ch_slot[c], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // for the next solve
+      }
+    }
+    return;
+  }
+  const int64_t nthreads = (int64_t)(gridDim.x - nchunk_blocks) * blockDim.x;
+  for (int64_t x = (int64_t)(blockIdx.x - nchunk_blocks) * blockDim.x + threadIdx.x; x < m; x += nthreads) {
+    if (is_long[x]) continue;
     double s[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) s[q] = in[x * NR + q];
@@ -165,8 +229,14 @@ static void tree_apply_nr(psg_ctx* c, const double* r, double* z) {
   int next = 0;
   for (int k = 0; k < t->rounds; ++k) {
     double* out = buf[next];
-    hipLaunchKernelGGL((k_tree_fwd<NR>), dim3(grid), dim3(kBlock), 0, s, m, t->cptr.p + (int64_t)k * (m + 1), t->cidx.p,
-                       t->wk.p + (int64_t)k * m, t->pivot.p, k == t->rounds - 1, in, out);
+    const int64_t c0 = t->chunk_off[k];
+    const int nch = (int)(t->chunk_off[k + 1] - c0);
+    const int nchunk_blocks = (nch + kBlock / kWave - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL((k_tree_fwd<NR>), dim3(nchunk_blocks + grid), dim3(kBlock), 0, s, m,
+                       t->cptr.p + (int64_t)k * (m + 1), t->cidx.p, t->is_long.p + (int64_t)k * m, nchunk_blocks, nch,
+                       t->ch_row.p + c0, t->ch_beg.p + c0, t->ch_end.p + c0, t->ch_first.p + c0, t->ch_count.p + c0,
+                       t->ch_slot.p + c0, t->ch_cnt.p, t->ch_part.p, t->wk.p + (int64_t)k * m, t->pivot.p,
+                       k == t->rounds - 1, in, out);
     HIP_LAUNCH_CHECK();
     in = out;
     next ^= 1;
@@ -275,6 +345,50 @@ int psg_tree_setup(psg_ctx* c, int32_t max_depth, const int32_t* parent, const i
     if (na) {
       t->arc_idx.upload(arc_idx, na, s);
       t->arc_coef.upload(arc_coef, na, s);
+    }
+    // chunks of the long child lists, per round; ch_first is relative to the
+    // round (the kernel gets the round's slice of the chunk arrays)
+    {
+      std::vector<uint8_t> is_long((size_t)std::max(rounds, 1) * m, 0);
+      std::vector<int> row, first, count, slot;
+      std::vector<int64_t> beg, end;
+      t->chunk_off.assign(rounds + 1, 0);
+      int nslot = 0;
+      size_t max_round = 1;
+      for (int k = 0; k < rounds; ++k) {
+        const int64_t* p = child_ptr + (int64_t)k * (m + 1);
+        const size_t round_start = row.size();
+        for (int x = 0; x < m; ++x) {
+          const int64_t len = p[x + 1] - p[x];
+          if (len <= kShort) continue;
+          is_long[(size_t)k * m + x] = 1;
+          const int nch = (int)((len + kChunk - 1) / kChunk);
+          const int f = (int)(row.size() - round_start);
+          for (int q = 0; q < nch; ++q) {
+            row.push_back(x);
+            beg.push_back(p[x] + (int64_t)q * kChunk);
+            end.push_back(std::min(p[x + 1], p[x] + (int64_t)(q + 1) * kChunk));
+            first.push_back(f);
+            count.push_back(nch);
+            slot.push_back(nslot);
+          }
+          ++nslot;
+        }
+        t->chunk_off[k + 1] = (int64_t)row.size();
+        max_round = std::max(max_round, row.size() - round_start);
+      }
+      t->is_long.upload(is_long, s);
+      auto up_i = [&](DevBuf<int>& b, std::vector<int>& v) { if (v.empty()) v.push_back(0); b.upload(v, s); };
+      auto up_l = [&](DevBuf<int64_t>& b, std::vector<int64_t>& v) { if (v.empty()) v.push_back(0); b.upload(v, s); };
+      up_i(t->ch_row, row);
+      up_i(t->ch_first, first);
+      up_i(t->ch_count, count);
+      up_i(t->ch_slot, slot);
+      up_l(t->ch_beg, beg);
+      up_l(t->ch_end, end);
+      std::vector<int> zeros(std::max(nslot, 1), 0);
+      t->ch_cnt.upload(zeros, s);
+      t->ch_part.alloc(max_round * kMaxRhs);
     }
     for (DevBuf<double>* b : {&t->terms, &t->edge, &t->pivot, &t->pclip, &t->mult}) b->alloc(m);
     t->wk.alloc((size_t)std::max(rounds, 1) * m);

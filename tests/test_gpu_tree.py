@@ -58,6 +58,7 @@ def test_tree_factor_and_apply_match_oracle(gpu, case):
         z = obj._apply(r)
         ref = T.tree_solve(topo, rp, rm, r)
         assert np.linalg.norm(z - ref) <= 1e-12 * np.linalg.norm(ref)
+        assert np.array_equal(obj._apply(r), z)  # fixed summation order, chunked lists included
     # two interleaved columns through the block apply
     R = np.ascontiguousarray(np.random.default_rng(30).standard_normal((m, 2)))
     Z = np.empty((m, 2))
