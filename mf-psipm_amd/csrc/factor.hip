@@ -828,7 +828,7 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
   __shared__ T cs[NR][kW];
   __shared__ T part_sh[NR][kBlock / kWave][kWave];
   __shared__ int tg[kBlock];
-  __shared__ int sh_u, sh_last;
+  __shared__ int sh_u;
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), wv = tid / kWave;
   int batch_next = 0, batch_end = 0;
@@ -851,6 +851,21 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
     const int64_t nr = a.rptr[s + 1] - a.rptr[s];
     const int nrow = a.su_n[u];
     const int64_t r0 = w + a.su_r0[u];
+    // The supernode's first block (its parent's rows, the last to become
+    // ready) finishes x_s: the other blocks add their column sums into acc
+    // and count themselves in pcnt, and take their tickets before it, so the
+    // finisher only waits on smaller tickets.  It adds its own sums from LDS
+    // to acc -- no atomics, no arrival count on the critical path -- and
+    // its block inverse and y_s, which do not depend on x, are fetched
+    // before the wait.  A single-unit supernode is its own finisher.
+    const int total = a.su_range[2 * s + 1] - a.su_range[2 * s];
+    const bool single = u == a.su_range[2 * s];  // the finisher
+    T ys[NR];
+    if (single) {
+      stage_inverse(sD, a.dinv + a.doff[s], w);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) ys[q] = tid < w ? a.y[(int64_t)(f + tid) * NR + q] : T(0);
+    }
     // wait for the ancestors owning this block's rows
     int row = -1;
     if (tid < nrow) {
@@ -876,10 +891,16 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
     }
     __syncthreads();
     if (tid < nrow && (tid == 0 || tg[tid - 1] != tg[tid])) wait_at_least(&a.xdone[tg[tid]], 1, a.err, a.sleep_max);
+    // the finisher also waits for the supernode's other blocks
+    if (single && total > 1 && tid == kBlock - 1) wait_at_least(&a.pcnt[s], total - 1, a.err, a.sleep_max);
     __syncthreads();
     if (a.trace && tid == 0) a.trace[6 * (int64_t)u + 4] = wall_clock64();
 #pragma unroll
     for (int q = 0; q < NR; ++q) xr[q][tid] = (tid < nrow) ? load_agent(a.x + (int64_t)row * NR + q) : T(0);
+    T others[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+      others[q] = (single && total > 1 && tid < w) ? load_agent(a.acc + (int64_t)(f + tid) * NR + q) : T(0);
     __syncthreads();
     // partial[c] = sum over the wave's rows of L[r][c] x[row]; the panel
     // block in registers serves every right-hand side
@@ -897,27 +918,14 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
       part_sh[q][wv][cg + 4 * rq] = transpose_reduce16(acc, lane);
     }
     __syncthreads();
-    if (tid < w && nrow > 0) {
-#pragma unroll
-      for (int q = 0; q < NR; ++q) {
-        const T sum = (part_sh[q][0][tid] + part_sh[q][1][tid]) + (part_sh[q][2][tid] + part_sh[q][3][tid]);
-        __hip_atomic_fetch_add(a.acc + (int64_t)(f + tid) * NR + q, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (tid == 0) {
-      const int total = a.su_range[2 * s + 1] - a.su_range[2 * s];
-      const int old = __hip_atomic_fetch_add(&a.pcnt[s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      sh_last = (old == total - 1);
-    }
-    __syncthreads();
-    if (sh_last) {
-      stage_inverse(sD, a.dinv + a.doff[s], w);
+    if (single) {
       if (tid < w) {
 #pragma unroll
-        for (int q = 0; q < NR; ++q)
-          cs[q][tid] = a.y[(int64_t)(f + tid) * NR + q] - load_agent(a.acc + (int64_t)(f + tid) * NR + q);
+        for (int q = 0; q < NR; ++q) {
+          const T sum = nrow > 0 ? (part_sh[q][0][tid] + part_sh[q][1][tid]) + (part_sh[q][2][tid] + part_sh[q][3][tid])
+                                 : T(0);
+          cs[q][tid] = ys[q] - (others[q] + sum);
+        }
       }
       __syncthreads();
       if (tid < w) {
@@ -931,7 +939,20 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       if (tid == 0) __hip_atomic_store(&a.xdone[s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (a.trace && tid == 0) a.trace[6 * (int64_t)u + 5] = wall_clock64();
+      __syncthreads();
+      continue;
     }
+    if (tid < w && nrow > 0) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        const T sum = (part_sh[q][0][tid] + part_sh[q][1][tid]) + (part_sh[q][2][tid] + part_sh[q][3][tid]);
+        __hip_atomic_fetch_add(a.acc + (int64_t)(f + tid) * NR + q, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // the column sums are performed before they are counted
+    __syncthreads();
+    if (tid == 0) __hip_atomic_fetch_add(&a.pcnt[s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (a.trace && tid == 0) a.trace[6 * (int64_t)u + 5] = wall_clock64();
     __syncthreads();
   }
@@ -2124,6 +2145,14 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
       const int a = us[x], b = us[y];
       return depth[a] < depth[b] || (depth[a] == depth[b] && level[a] > level[b]);
     });
+    // a supernode's units stay adjacent (equal keys keep their order); its
+    // first block finishes x_s after the others, so it takes the last ticket
+    for (size_t q = 0; q < border.size();) {
+      size_t e = q + 1;
+      while (e < border.size() && us[border[e]] == us[border[q]]) ++e;
+      if (e - q > 1) std::rotate(border.begin() + q, border.begin() + q + 1, border.begin() + e);
+      q = e;
+    }
     fs->bwd_order.upload(border, s);
     fs->su_s.upload(us, s);
     fs->su_r0.upload(ur0, s);
