@@ -35,7 +35,7 @@ namespace psg {
 constexpr int kW = 64;      // max supernode width handled by the panel kernels
 constexpr int kRB = 256;    // rows per TRSM / solve unit
 constexpr int kTile = 64;   // update tile edge
-constexpr int kSC = 64;     // max columns per symmetric dense-tail tile
+constexpr int kSC = 256;    // max columns per symmetric dense-tail tile
 
 struct LevelRange {
   int node_begin, node_end;   // into level_nodes
@@ -1677,16 +1677,19 @@ static void setup_tail(psg_ctx* c, const Plan& P) {
     fs->tail_ld = (r + 3) / 4 * 4;
     // one vector per lane (512-row FP64 tiles) measured fastest: more
     // resident waves beat more loads per wave; PSG_TAIL_KV = 2 or 4 widens
-    int sc = kSC, kv = 1;
+    int sc = 64, kv = 1;
     const char* kvs = getenv("PSG_TAIL_KV");
     if (kvs) kv = atoi(kvs) >= 4 ? 4 : (atoi(kvs) >= 2 ? 2 : 1);
+    if (const char* e = getenv("PSG_TAIL_SC")) sc = std::max(16, std::min(kSC, atoi(e)));
+    int64_t min_tiles = 1024;
+    if (const char* e = getenv("PSG_TAIL_TILES")) min_tiles = std::max(1, atoi(e));
     auto ntiles = [&](int sc_, int kv_) {
       const int rt = kBlock * V * kv_;
       int64_t n = 0;
       for (int J = 0; J * sc_ < r; ++J) n += (r + rt - 1) / rt - (J * sc_) / rt;
       return n;
     };
-    while (ntiles(sc, kv) < 1024 && (kv > 1 || sc > 16)) {
+    while (ntiles(sc, kv) < min_tiles && (kv > 1 || sc > 16)) {
       if (kv > 1) kv >>= 1;
       else sc >>= 1;
     }
