@@ -321,11 +321,47 @@ k_pcg_direction(int m, const double* __restrict__ partC, int nbC, const double* 
 // vector update is a no-op while kHold or kStop is set.  The arithmetic and
 // its order are those of the host-judged loop.
 
+// Sum of the nb (<= kRedBlocks) block partials src[0..nb) in index order --
+// the host's fixed order -- by thread 0 of the block; the block first stages
+// the partials in LDS with coalesced agent-scope loads (they were written by
+// other workgroups of the same launch), so the sequential sum waits on LDS
+// rather than on nb dependent global loads.
+__device__ inline double ordered_partial_sum(const double* __restrict__ src, int nb) {
+  __shared__ double sp[kRedBlocks];
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    sp[b] = __hip_atomic_load(const_cast<double*>(src + b), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  double rr = 0.0;
+  if (threadIdx.x == 0)
+    for (int b = 0; b < nb; ++b) rr += sp[b];
+  __syncthreads();
+  return rr;
+}
+
+// A block's partial is published with an agent-scope store and counted;
+// returns true (in every thread) in the last block to arrive, which then
+// sees every partial.  The counter is reset there for the next launch.
+__device__ inline bool publish_and_count(double* __restrict__ slot, const double* vals, int nv, int stride,
+                                         int* __restrict__ ctr) {
+  __shared__ int sh_last;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < nv; ++k)
+      __hip_atomic_store(slot + (size_t)k * stride, vals[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);  // the partials are performed before they are counted
+    const int old = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sh_last = old == (int)gridDim.x - 1;
+    if (sh_last) __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  return sh_last != 0;
+}
+
 // alpha = rz / (p.q) with the guards of SSP:490-501, x += alpha p, r -= alpha q
 __global__ void __launch_bounds__(kBlock)
 k_pcg_step_d(int m, const double* __restrict__ hv_dots, const double* __restrict__ rzs,
              const double* __restrict__ p, const double* __restrict__ q, double* __restrict__ x,
-             double* __restrict__ r, double* __restrict__ scal, double* __restrict__ alphas) {
+             double* __restrict__ r, double* __restrict__ scal, double* __restrict__ alphas,
+             int* __restrict__ ctr) {
   if (scal[kStop] != 0.0 || scal[kHold] != 0.0) return;
   const double curvature = hv_dots[0];
   const double floor_ = std::numeric_limits<double>::min() * fmax(1.0, sqrt(hv_dots[1]) * sqrt(hv_dots[2]));
@@ -351,38 +387,17 @@ k_pcg_step_d(int m, const double* __restrict__ hv_dots, const double* __restrict
     acc[0] += ri * ri;
   }
   block_sum<1>(acc);
-  if (threadIdx.x == 0) {
-    scal[kScalarHeader + blockIdx.x] = acc[0];
-    if (blockIdx.x == 0) {
-      scal[kAlpha] = alpha;
-      scal[kCurv] = curvature;
-      scal[kRzUsed] = rz;
-      alphas[(int)scal[kIt]] = alpha;
-    }
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    scal[kAlpha] = alpha;
+    scal[kCurv] = curvature;
+    scal[kRzUsed] = rz;
+    alphas[(int)scal[kIt]] = alpha;
   }
-}
-
-// one thread: count the step, keep |r|^2 and hold for the host when the
-// recurrence meets the tolerance, a replacement is due or maxit is reached
-// Sum of the nb (<= kRedBlocks) block partials src[0..nb) in index order --
-// the host's fixed order -- by thread 0 of a one-wave block; the wave first
-// stages the partials in LDS with coalesced loads, so the sequential sum
-// waits on LDS rather than on nb dependent global loads.
-__device__ inline double ordered_partial_sum(const double* __restrict__ src, int nb) {
-  __shared__ double sp[kRedBlocks];
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) sp[b] = src[b];
-  __syncthreads();
-  double rr = 0.0;
-  if (threadIdx.x == 0)
-    for (int b = 0; b < nb; ++b) rr += sp[b];
-  __syncthreads();
-  return rr;
-}
-
-__global__ void k_pcg_judge(int nb, double* __restrict__ scal) {
-  if (blockIdx.x != 0) return;
-  if (scal[kStop] != 0.0 || scal[kHold] != 0.0) return;  // uniform over the block
-  const double rr = ordered_partial_sum(scal + kScalarHeader, nb);
+  if (!publish_and_count(scal + kScalarHeader + blockIdx.x, acc, 1, 0, ctr)) return;
+  // the last block judges: count the step, keep |r|^2 and hold for the host
+  // when the recurrence meets the tolerance, a replacement is due or maxit
+  // is reached
+  const double rr = ordered_partial_sum(scal + kScalarHeader, gridDim.x);
   if (threadIdx.x != 0) return;
   const int it = (int)scal[kIt] + 1;
   scal[kIt] = it;
@@ -391,6 +406,7 @@ __global__ void k_pcg_judge(int nb, double* __restrict__ scal) {
   const double rec = sqrt(rr) / scal[kScale];
   if ((repl > 0 && it % repl == 0) || rec <= scal[kTol] || it >= (int)scal[kMaxit]) scal[kHold] = 1.0;
 }
+
 
 // beta = rz_new / rz (guards of SSP:545-553), p = z + beta p
 __global__ void __launch_bounds__(kBlock)
@@ -700,6 +716,8 @@ int psg_create(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowin
       c->partD.alloc(3 * kSpmvBlocks);
       c->hvdots.alloc(4);
       c->scal.alloc(kScalarHeader + kRedBlocks);
+      c->pcg_ctr.alloc(2);
+      HIP_CHECK(hipMemsetAsync(c->pcg_ctr.p, 0, 2 * sizeof(int), c->stream));
       HIP_CHECK(hipHostMalloc((void**)&c->hscal, (kScalarHeader + kRedBlocks) * sizeof(double)));
       HIP_CHECK(hipStreamSynchronize(c->stream));
     } catch (...) {
@@ -886,8 +904,7 @@ int psg_pcg(psg_ctx* c, double tol, int32_t maxit, int32_t replacement, double g
     auto iteration = [&]() {
       device_Hv(c, c->p.p, c->q.p, c->partA.p, true, c->hvdots.p);
       hipLaunchKernelGGL(k_pcg_step_d, dim3(grid_m), dim3(kBlock), 0, s, m, c->hvdots.p, c->rzs.p, c->p.p, c->q.p,
-                         c->x.p, c->r.p, c->scal.p, c->alphas.p);
-      hipLaunchKernelGGL(k_pcg_judge, dim3(1), dim3(kWave), 0, s, grid_m, c->scal.p);
+                         c->x.p, c->r.p, c->scal.p, c->alphas.p, c->pcg_ctr.p);
       factor_apply(c, c->r.p, c->z.p);
       const int nbC = device_dot(c, c->r.p, c->z.p, m, c->partC.p);
       hipLaunchKernelGGL(k_pcg_direction_d, dim3(grid_m), dim3(kBlock), 0, s, m, c->partC.p, nbC, c->rzs.p, c->z.p,
@@ -1250,12 +1267,14 @@ k_pcg_direction2(int m, const double* __restrict__ partC, int nbC, int mode0, in
   }
 }
 
-// ---- device-judged two-column loop (as k_pcg_step_d / k_pcg_judge /
-// k_pcg_direction_d, per column; the hold flag lives in column 0's block)
+// ---- device-judged two-column loop (as k_pcg_step_d with its judging last
+// block, and k_pcg_direction_d, per column; the hold flag lives in column
+// 0's block)
 __global__ void __launch_bounds__(kBlock)
 k_pcg_step2_d(int m, const double* __restrict__ hv, const double* __restrict__ rzs, int cap,
               const double* __restrict__ p, const double* __restrict__ q, double* __restrict__ x,
-              double* __restrict__ r, double* __restrict__ scal, double* __restrict__ alphas) {
+              double* __restrict__ r, double* __restrict__ scal, double* __restrict__ alphas,
+              int* __restrict__ ctr) {
   const bool hold = scal[kHold] != 0.0;
   double alpha[2] = {0.0, 0.0}, rzv[2] = {0.0, 0.0};
   bool live[2];
@@ -1302,33 +1321,28 @@ k_pcg_step2_d(int m, const double* __restrict__ hv, const double* __restrict__ r
     }
   }
   block_sum<2>(acc);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
 #pragma unroll
-    for (int c = 0; c < 2; ++c) scal[2 * kScalarHeader + c * gridDim.x + blockIdx.x] = acc[c];
-    if (blockIdx.x == 0) {
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        if (!live[c]) continue;
-        double* h = scal + c * kScalarHeader;
-        h[kAlpha] = alpha[c];
-        h[kCurv] = hv[3 * c];
-        h[kRzUsed] = rzv[c];
-        alphas[(int64_t)c * cap + (int)h[kIt]] = alpha[c];
-      }
+    for (int c = 0; c < 2; ++c) {
+      if (!live[c]) continue;
+      double* h = scal + c * kScalarHeader;
+      h[kAlpha] = alpha[c];
+      h[kCurv] = hv[3 * c];
+      h[kRzUsed] = rzv[c];
+      alphas[(int64_t)c * cap + (int)h[kIt]] = alpha[c];
     }
   }
-}
-
-__global__ void k_pcg_judge2(int nb, double* __restrict__ scal) {
-  if (blockIdx.x != 0) return;
-  if (scal[kHold] != 0.0) return;  // uniform over the block
+  const int nb = gridDim.x;
+  if (!publish_and_count(scal + 2 * kScalarHeader + blockIdx.x, acc, 2, nb, ctr)) return;
+  // the last block judges each column that stepped (its own live flags are
+  // every block's: the same inputs, the same guards)
   double rrs[2];
   for (int c = 0; c < 2; ++c) rrs[c] = ordered_partial_sum(scal + 2 * kScalarHeader + c * nb, nb);
   if (threadIdx.x != 0) return;
   bool any = false;
   for (int c = 0; c < 2; ++c) {
     double* h = scal + c * kScalarHeader;
-    if (h[kStop] != 0.0) continue;
+    if (!live[c]) continue;
     const double rr = rrs[c];
     const int it = (int)h[kIt] + 1;
     h[kIt] = it;
@@ -1537,8 +1551,7 @@ int psg_pcg2(psg_ctx* c, const double* tol, int32_t maxit, int32_t replacement, 
     auto iteration = [&]() {
       device_Hv2(c, b, b.p.p, b.q.p, b.partA.p, true, b.hvdots.p);
       hipLaunchKernelGGL(k_pcg_step2_d, dim3(grid_m), dim3(kBlock), 0, s, m, b.hvdots.p, b.rzs.p, cap, b.p.p, b.q.p,
-                         b.x.p, b.r.p, b.scal.p, b.alphas.p);
-      hipLaunchKernelGGL(k_pcg_judge2, dim3(1), dim3(kWave), 0, s, grid_m, b.scal.p);
+                         b.x.p, b.r.p, b.scal.p, b.alphas.p, c->pcg_ctr.p + 1);
       factor_apply2(c, b.r.p, b.z.p);
       const int nbC = device_dot2(c, b.r.p, b.z.p, b.partC.p);
       hipLaunchKernelGGL(k_pcg_direction2_d, dim3(grid_m), dim3(kBlock), 0, s, m, b.partC.p, nbC, b.rzs.p, cap,

@@ -109,6 +109,9 @@ struct psg_ctx {
   psg::DevBuf<double> partA, partC, partD;  // per-block partials
   psg::DevBuf<double> hvdots;               // p.q, p.p, q.q of the last H p
   psg::DevBuf<double> scal;                 // header + partials of the axpy kernel
+  // arrival counters of the device-judged step kernels ([0] single column,
+  // [1] pair): the last block to finish judges; it resets its counter
+  psg::DevBuf<int> pcg_ctr;
   psg::DevBuf<double> rzs, alphas, betas;
   double* hscal = nullptr;                  // pinned mirror of scal
   bool have_rhs = false, have_solution = false;

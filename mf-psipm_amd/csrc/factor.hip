@@ -105,6 +105,12 @@ struct FactorState {
   DevBuf<int> tail_sym_tiles;
   DevBuf<double> tail_Q64;
   DevBuf<float> tail_Q32;
+  // the tail spans every row (small LPs): k_dense_apply does the whole apply,
+  // accumulating in dense_acc (cleared by its last workgroup)
+  bool dense_all = false;
+  DevBuf<double> dense_acc64;
+  DevBuf<float> dense_acc32;
+  DevBuf<int> dense_ctr;
   struct TailStep { int kind, j, w; MGemm g; };  // kind: 0 potrf, 1 trsm, 2 update
   std::vector<TailStep> tail_steps;
   struct InvLaunch { int tile_begin, ntile; };
@@ -1508,10 +1514,11 @@ k_tail_upper_vec(int r, int ur, const int* __restrict__ tiles, const T* __restri
 // lane KV vectors of V rows.  Entry (i, j) with i > j adds Q_ij b_j to x_i
 // (kept per lane over the tile's columns) and Q_ij b_i to x_j (summed over
 // the wave, then over the four waves in LDS); the diagonal adds once.
-template <class T, int NR, int KV>
-__global__ void __launch_bounds__(kBlock)
-k_tail_symv(int r, int ld, int sc, const int* __restrict__ tiles, const T* __restrict__ Q, const T* __restrict__ b,
-            T* __restrict__ x, int* __restrict__ done, int ndone) {
+// One tile of the symmetric product; bload(k, e) is entry (k, e) of the
+// right-hand side (NR interleaved columns).
+template <class T, int NR, int KV, class BL>
+__device__ inline void symv_tile(int r, int ld, int sc, int J, int I, const T* __restrict__ Q, const BL& bload,
+                                 T* __restrict__ x) {
   typedef typename Vec16<T>::v VT;
   constexpr int V = Vec16<T>::n;
   constexpr int RW = kWave * V * KV;         // rows per wave
@@ -1519,15 +1526,11 @@ k_tail_symv(int r, int ld, int sc, const int* __restrict__ tiles, const T* __res
   constexpr int RT = RW * NW;                // rows per tile
   __shared__ double bj_s[NR][kSC];
   __shared__ double cpart[NW][NR][kSC];
-  const int J = tiles[2 * blockIdx.x], I = tiles[2 * blockIdx.x + 1];
   const int j0 = J * sc, i0 = I * RT;
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-  // the tail supernodes count as solved for the backward launch that follows
-  if (blockIdx.x == 0)
-    for (int q = threadIdx.x; q < ndone; q += blockDim.x) done[q] = 1;
   for (int q = threadIdx.x; q < sc; q += blockDim.x) {
 #pragma unroll
-    for (int e = 0; e < NR; ++e) bj_s[e][q] = (j0 + q < r) ? (double)b[(int64_t)(j0 + q) * NR + e] : 0.0;
+    for (int e = 0; e < NR; ++e) bj_s[e][q] = (j0 + q < r) ? bload(j0 + q, e) : 0.0;
   }
   const int iw = i0 + wv * RW + lane * V;  // first row of this lane's vector k is iw + k kWave V
   double bi[KV][V][NR], racc[KV][V][NR];
@@ -1538,34 +1541,35 @@ k_tail_symv(int r, int ld, int sc, const int* __restrict__ tiles, const T* __res
       const int row = iw + k * kWave * V + t;
 #pragma unroll
       for (int e = 0; e < NR; ++e) {
-        bi[k][t][e] = row < r ? (double)b[(int64_t)row * NR + e] : 0.0;
+        bi[k][t][e] = row < r ? bload(row, e) : 0.0;
         racc[k][t][e] = 0.0;
       }
     }
   __syncthreads();
   const int jend = min(j0 + sc, r);
-  // two columns per step: both columns' loads are in flight together, and
-  // one exchange across the wave halves starts both column reductions
-  for (int j = j0; j < jend; j += 2) {
-    const bool two = j + 1 < jend;  // uniform
-    VT v[2][KV];
+  // four columns per step: their loads are in flight together, and two
+  // exchanges (across the wave halves, then the quarter-waves) start all four
+  // column reductions; lanes 0, 32, 16, 48 end with columns j .. j + 3
+  constexpr int CG = 4;
+  for (int j = j0; j < jend; j += CG) {
+    VT v[CG][KV];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < CG; ++h) {
       const int jj = j + h;
       const T* col = Q + (int64_t)jj * ld;
 #pragma unroll
       for (int k = 0; k < KV; ++k) {
         const int i = iw + k * kWave * V;
-        if ((h == 0 || two) && i + V > jj && i < r) v[h][k] = *reinterpret_cast<const VT*>(col + i);
+        if (jj < jend && i + V > jj && i < r) v[h][k] = *reinterpret_cast<const VT*>(col + i);
       }
     }
-    double cacc[2][NR];
+    double cacc[CG][NR];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < CG; ++h) {
 #pragma unroll
       for (int e = 0; e < NR; ++e) cacc[h][e] = 0.0;
-      if (h == 1 && !two) continue;
       const int jj = j + h;
+      if (jj >= jend) continue;  // uniform
       double bj[NR];
 #pragma unroll
       for (int e = 0; e < NR; ++e) bj[e] = bj_s[e][jj - j0];
@@ -1586,16 +1590,20 @@ k_tail_symv(int r, int ld, int sc, const int* __restrict__ tiles, const T* __res
         }
       }
     }
-    // lanes < 32 keep column j and take the upper half's column-j partial,
-    // lanes >= 32 the reverse; lane 0 ends with column j, lane 32 with j + 1
-    const bool hi = lane & 32;
+    const bool hi32 = lane & 32, hi16 = lane & 16;
 #pragma unroll
     for (int e = 0; e < NR; ++e) {
-      double sm = (hi ? cacc[1][e] : cacc[0][e]) + __shfl_xor(hi ? cacc[0][e] : cacc[1][e], 32, kWave);
+      // lanes < 32 keep columns 0 and 2, lanes >= 32 columns 1 and 3
+      const double a0 = (hi32 ? cacc[1][e] : cacc[0][e]) + __shfl_xor(hi32 ? cacc[0][e] : cacc[1][e], 32, kWave);
+      const double a1 = (hi32 ? cacc[3][e] : cacc[2][e]) + __shfl_xor(hi32 ? cacc[2][e] : cacc[3][e], 32, kWave);
+      // quarter-waves with lane bit 4 clear keep a0, the others a1
+      double sm = (hi16 ? a1 : a0) + __shfl_xor(hi16 ? a0 : a1, 16, kWave);
 #pragma unroll
-      for (int o = 16; o > 0; o >>= 1) sm += __shfl_xor(sm, o, kWave);
-      if (lane == 0) cpart[wv][e][j - j0] = sm;
-      if (lane == 32 && two) cpart[wv][e][j + 1 - j0] = sm;
+      for (int o = 8; o > 0; o >>= 1) sm += __shfl_xor(sm, o, kWave);
+      if ((lane & 15) == 0) {
+        const int h = (hi32 ? 1 : 0) + (hi16 ? 2 : 0);
+        if (j + h < jend) cpart[wv][e][j + h - j0] = sm;
+      }
     }
   }
   __syncthreads();
@@ -1619,6 +1627,66 @@ k_tail_symv(int r, int ld, int sc, const int* __restrict__ tiles, const T* __res
                                  __HIP_MEMORY_SCOPE_AGENT);
       }
     }
+}
+
+template <class T, int NR, int KV>
+__global__ void __launch_bounds__(kBlock)
+k_tail_symv(int r, int ld, int sc, const int* __restrict__ tiles, const T* __restrict__ Q, const T* __restrict__ b,
+            T* __restrict__ x, int* __restrict__ done, int ndone) {
+  // the tail supernodes count as solved for the backward launch that follows
+  if (blockIdx.x == 0)
+    for (int q = threadIdx.x; q < ndone; q += blockDim.x) done[q] = 1;
+  auto bload = [&](int k, int e) { return (double)b[(int64_t)k * NR + e]; };
+  symv_tile<T, NR, KV>(r, ld, sc, tiles[2 * blockIdx.x], tiles[2 * blockIdx.x + 1], Q, bload, x);
+}
+
+// The whole apply of a factor that is one dense block plus isolated 1x1
+// supernodes (the tail spans every other row, as on small LPs):
+// z = D P' L^-T L^-1 P D r in one launch.  Tiles read P D r through the
+// permutation (tail rows F + k), add into acc, and the last tile to finish
+// (by an arrival counter) writes z = D P' acc, solves the isolated rows
+// (x_k = l_kk^-2 b_k) and clears acc and the counter.
+template <class T, int NR, int KV>
+__global__ void __launch_bounds__(kBlock)
+k_dense_apply(int r, int F, int ld, int sc, const int* __restrict__ tiles, const T* __restrict__ Q,
+              const int* __restrict__ perm, const double* __restrict__ dsc, const double* __restrict__ rin,
+              int n_iso, const int* __restrict__ iso_col, const int* __restrict__ iso_doff,
+              const T* __restrict__ dinv, T* __restrict__ acc, int* __restrict__ ctr, double* __restrict__ z) {
+  auto bload = [&](int k, int e) {
+    const int i = perm[F + k];
+    return (double)(T)(dsc[i] * rin[(int64_t)i * NR + e]);
+  };
+  symv_tile<T, NR, KV>(r, ld, sc, tiles[2 * blockIdx.x], tiles[2 * blockIdx.x + 1], Q, bload, acc);
+  __shared__ int sh_last;
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's additions are performed before the barrier
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int old = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sh_last = old == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!sh_last) return;
+  for (int k = threadIdx.x; k < r; k += blockDim.x) {
+    const int i = perm[F + k];
+    const double d = dsc[i];
+#pragma unroll
+    for (int e = 0; e < NR; ++e) {
+      T* a = acc + (int64_t)k * NR + e;
+      z[(int64_t)i * NR + e] = d * (double)__hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(a, T(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  for (int t = threadIdx.x; t < n_iso; t += blockDim.x) {
+    const int i = perm[iso_col[t]];
+    const double d = dsc[i];
+    const T di = dinv[iso_doff[t]];
+#pragma unroll
+    for (int e = 0; e < NR; ++e) {
+      const T bk = (T)(d * rin[(int64_t)i * NR + e]);
+      z[(int64_t)i * NR + e] = d * (double)(di * (di * bk));
+    }
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void k_fill_int(int* __restrict__ p, int n, int v) {
@@ -2163,6 +2231,7 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
     fs->su_range.upload(range, s);
     fs->need.upload(need, s);
     fs->flags.alloc(4 * (size_t)P.nsuper + 8);
+    zero_async(fs->flags.p, (int64_t)fs->flags.n, s);
     if (c->precision == PSG_MIXED) {
       fs->sf_part32.alloc((size_t)fs->sf_units * kW);
       fs->sf_part64.release();
@@ -2180,6 +2249,21 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
     fs->sf_batch = knob("PSG_SF_BATCH", 1, 1, 64);
     if (getenv("PSG_SF_TRACE")) fs->trace.alloc(6 * (size_t)fs->sf_units);
     else fs->trace.release();
+    // one dense block spans the whole factor: the apply is one launch
+    fs->dense_all = fs->tail_nchunk && fs->tail_sym && fs->tail_r + fs->n_iso == P.m && fs->sf_units == 0 &&
+                    fs->low_ptr.size() <= 1 && !getenv("PSG_DENSE_APPLY_OFF");
+    if (fs->dense_all) {
+      const size_t na = (size_t)fs->tail_r * kMaxRhs;
+      if (c->precision == PSG_MIXED) {
+        fs->dense_acc32.alloc(na);
+        zero_async(fs->dense_acc32.p, (int64_t)na, s);
+      } else {
+        fs->dense_acc64.alloc(na);
+        zero_async(fs->dense_acc64.p, (int64_t)na, s);
+      }
+      fs->dense_ctr.alloc(1);
+      zero_async(fs->dense_ctr.p, 1, s);
+    }
   }
   fs->perm.upload(P.perm, s);
   fs->sfirst.upload(P.sfirst, s);
@@ -2344,6 +2428,33 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   hipStream_t s = c->stream;
   const int m = c->m;
   const int ns = fs->nsuper;
+  if (fs->dense_all) {
+    const T* Q = nullptr;
+    T* acc = nullptr;
+    if constexpr (sizeof(T) == 8) {
+      Q = fs->tail_Q64.p;
+      acc = fs->dense_acc64.p;
+    } else {
+      Q = fs->tail_Q32.p;
+      acc = fs->dense_acc32.p;
+    }
+    const dim3 grid(fs->tail_nsym), blk(kBlock);
+    const int ld = fs->tail_ld, sc = fs->tail_sc;
+    const int* tl = fs->tail_sym_tiles.p;
+    const int rt = fs->tail_r, F = fs->tail_F, ni = fs->n_iso;
+    const int *ic = fs->iso_col.p, *io = fs->iso_doff.p;
+    if (fs->tail_kv == 4)
+      hipLaunchKernelGGL((k_dense_apply<T, NR, 4>), grid, blk, 0, s, rt, F, ld, sc, tl, Q, fs->perm.p, c->dsc.p, r, ni,
+                         ic, io, dinv, acc, fs->dense_ctr.p, z);
+    else if (fs->tail_kv == 2)
+      hipLaunchKernelGGL((k_dense_apply<T, NR, 2>), grid, blk, 0, s, rt, F, ld, sc, tl, Q, fs->perm.p, c->dsc.p, r, ni,
+                         ic, io, dinv, acc, fs->dense_ctr.p, z);
+    else
+      hipLaunchKernelGGL((k_dense_apply<T, NR, 1>), grid, blk, 0, s, rt, F, ld, sc, tl, Q, fs->perm.p, c->dsc.p, r, ni,
+                         ic, io, dinv, acc, fs->dense_ctr.p, z);
+    HIP_LAUNCH_CHECK();
+    return;
+  }
   // counters, column sums and tail targets are left cleared by the previous
   // solve's last kernel; cleared here only after a new factor or a failure
   const bool clean = fs->sf_clean == NR;
@@ -2436,7 +2547,8 @@ int* factor_knob(psg_ctx* c, int knob) {
 
 int factor_solve_error(psg_ctx* c) {
   FactorState* fs = c->fs;
-  if (c->precond == 1 || !fs || fs->solve_mode == 0 || !fs->flags.p) return 0;  // the tree solve has no waits
+  // the tree solve and the one-launch dense apply have no dependency waits
+  if (c->precond == 1 || !fs || fs->solve_mode == 0 || fs->dense_all || !fs->flags.p) return 0;
   int e = 0;
   HIP_CHECK(hipMemcpyAsync(&e, fs->flags.p + 4 * fs->nsuper + 4, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));

@@ -305,6 +305,54 @@ def test_dense_tail_inverse_solve(gpu, monkeypatch, precision, m, n, k, frac, ta
     assert np.linalg.norm(s1 - ref) <= (1e-9 if precision == "fp64" else 5e-6 * cond) * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("core_kind", ["all", "subset"])
+def test_single_launch_dense_apply_matches_split_apply(gpu, monkeypatch, precision, core_kind):
+    """When the dense tail spans every row (small LPs) the whole apply is one
+    launch (permutation, symmetric product, permutation back); it equals the
+    three-launch apply and LAPACK, for one and two columns, and stays
+    correct on repeated calls (its accumulator is cleared by the launch)."""
+    _say(f"single-launch dense apply {precision} core={core_kind}")
+    _, lp = _lp(1200, 3600, 5, 81)
+    A = lp.A
+    cand = np.flatnonzero(np.diff(A.indptr) >= 2)
+    # all: every row touched (one tree, one dense block); subset: rows no
+    # core column touches stay isolated 1x1 supernodes beside the block
+    core = cand if core_kind == "all" else np.sort(np.random.default_rng(82).choice(cand, 1200, replace=False))
+    theta = _theta(3600, 83, spread=2.0)
+    monkeypatch.setenv("PSG_TAIL_MIN", "64")
+    monkeypatch.setenv("PSG_TAIL_SYM", "1")
+    outs = []
+    for off in ("", "1"):
+        if off:
+            monkeypatch.setenv("PSG_DENSE_APPLY_OFF", "1")
+        obj = HipSpectralStableColumnPreconditioner(A, protected=~lp.pos, precision=precision)
+        obj._call("psg_set_weights", obj._ctx, theta, 1e-2)
+        plan = SymbolicPlan(A, core)
+        assert obj._call("psg_factor", obj._ctx, plan.handle, 0.0, None) == 0
+        info = np.zeros(8)
+        obj._call("psg_factor_info", obj._ctx, info)
+        R = np.ascontiguousarray(np.random.default_rng(84).standard_normal((1200, 2)))
+        singles = [obj._apply(R[:, 0]) for _ in range(3)] + [obj._apply(R[:, 1])]
+        Z = np.empty((1200, 2))
+        obj._call("psg_apply_P2", obj._ctx, R, Z)
+        outs.append((info, singles, Z.copy()))
+        plan.close()
+        obj.close()
+    (i1, s1, z1), (i2, s2, z2) = outs
+    assert i1[3] + i1[2] == 1200 and i1[6] == 1  # tail + isolated rows span every row, symmetric product
+    assert (i1[2] == 0) == (core_kind == "all")
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    for a, b in ((s1[0], s2[0]), (s1[3], s2[3]), (z1[:, 0], s2[0]), (z1[:, 1], s2[3]), (z1, z2)):
+        assert np.linalg.norm(a - b) <= tol * np.linalg.norm(b)
+    # repeated calls agree (the accumulator is cleared; atomics fix no summation order)
+    for k in (1, 2):
+        assert np.linalg.norm(s1[k] - s1[0]) <= tol * np.linalg.norm(s1[0])
+    apply_ref, cond = _dense_precond(A, core, 1.0 / theta, 1e-2)
+    ref = apply_ref(R[:, 0])
+    assert np.linalg.norm(s1[0] - ref) <= (1e-9 if precision == "fp64" else 5e-6 * cond) * np.linalg.norm(ref)
+
+
 @pytest.mark.parametrize("nsplit", [2, 3, 7])
 def test_apply_H_column_blocked_csr(gpu, monkeypatch, nsplit):
     _say(f"apply_H split={nsplit}")
