@@ -99,6 +99,8 @@ def gpu_leg(lp, args, precision):
     solver._call("psg_time_kernels", solver._ctx, 20, kt)
     finfo = np.zeros(8)
     solver._call("psg_factor_info", solver._ctx, finfo)
+    tail_ms = C.c_double(-1.0)
+    solver._call("psg_time_tail", solver._ctx, 20, C.byref(tail_ms))
     # two right-hand sides in one block PCG (every read of A and L serves both)
     pair = None
     if not args.no_pair:
@@ -121,7 +123,7 @@ def gpu_leg(lp, args, precision):
         nnz_L=int(solver.nnz_factor), plan=dict(solver._plan.info), hv_ms=kt[0], hv_loop_ms=kt[1],
         apply_ms=kt[2], dot_ms=kt[3], theta=theta, r2=r2, core_cols=solver.core.copy(), pair=pair,
         tuned=tuned, tail=dict(r=int(finfo[3]), supernodes=int(finfo[4]), symmetric=int(finfo[6]),
-                               panel_entries=int(finfo[7])),
+                               panel_entries=int(finfo[7])), tail_ms=tail_ms.value,
     )
     solver.close()
     return out
@@ -399,6 +401,10 @@ def main():
     bytes_apply = (2 * (g64["plan"]["nnz_panels"] - tail["panel_entries"]) * 8 + tail_reads
                    + 2 * g64["plan"]["row_structure"] * 4 + 8 * 8 * m)
     achieved_apply = bytes_apply / (g64["apply_ms"] * 1e-3) / 1e9
+    # the largest single kernel of an iteration: the dense tail's symmetric
+    # product (Q's lower triangle read once, b read and x written once)
+    bytes_tail = (1 if tail["symmetric"] else 2) * (tr * (tr + 1) // 2) * 8 + 16 * tr
+    achieved_tail = bytes_tail / (g64["tail_ms"] * 1e-3) / 1e9 if g64["tail_ms"] > 0 else None
     line = {
         "metric": "PCG iterations/s (plain FP64 GPU path), 1 MI355X",
         "value": round(g64["it_per_s"], 3),
@@ -422,10 +428,16 @@ def main():
         "kernel_ms": {"Hv": g64["hv_ms"], "apply_P": g64["apply_ms"], "dot": g64["dot_ms"],
                       "Hv_mixed": None if mixed is None else mixed["hv_loop_ms"],
                       "apply_P_mixed": None if mixed is None else mixed["apply_ms"]},
-        "roofline": {"bound": "hbm", "kernel": "P^-1 r: k_fwd_low + k_fwd_sf + k_tail_symv + k_bwd_sf + k_bwd_low (triangular solves around the dense tail's symmetric product)",
-                     "achieved": round(achieved_apply, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved_apply / HBM_PEAK_GBS, 4), "traffic": traffic.get("apply_P_bytes"),
-                     "bytes_per_launch": int(bytes_apply), "ms_per_launch": round(g64["apply_ms"], 4)},
+        "roofline": None if achieved_tail is None else {
+            "bound": "hbm", "kernel": "k_tail_symv: x_T = (L_T L_T')^-1 b_T, the dense tail of P^-1 r "
+                                      f"(r = {tr}), the largest kernel of an iteration",
+            "achieved": round(achieved_tail, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved_tail / HBM_PEAK_GBS, 4), "traffic": traffic.get("tail_bytes"),
+            "bytes_per_launch": int(bytes_tail), "ms_per_launch": round(g64["tail_ms"], 4)},
+        "roofline_apply": {"bound": "hbm", "kernel": "P^-1 r: k_perm_in_iso + k_fwd_low + k_fwd_sf + k_tail_symv + k_bwd_sf + k_bwd_low + k_perm_out_reset (the whole apply)",
+                           "achieved": round(achieved_apply, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved_apply / HBM_PEAK_GBS, 4), "traffic": traffic.get("apply_P_bytes"),
+                           "bytes_per_launch": int(bytes_apply), "ms_per_launch": round(g64["apply_ms"], 4)},
         "roofline_hv": {"bound": "hbm", "kernel": "H v: k_seg_gather + k_csr (+finalize)",
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get("Hv_bytes"),

@@ -254,6 +254,11 @@ int psg_dense_cholinv(psg_ctx* ctx, int64_t n, double* a);
  * dense-tail supernodes}.
  * No counterpart in the reference. */
 int psg_factor_info(psg_ctx* ctx, double* out);
+/* Device time (ms, HIP events on the context's stream) of the dense tail's
+ * product alone -- k_tail_symv, or the two triangular products -- averaged
+ * over reps launches; *ms = -1 without a dense tail.  No counterpart in the
+ * reference; used to price the hot path's largest kernel. */
+int psg_time_tail(psg_ctx* ctx, int32_t reps, double* ms);
 /* ---- learned column selector ---------------------------------------------
  * No counterpart in the reference, whose core is chosen by the leverage rule
  * alone (SAP:293-371, replaced on the host by psg_scores + the same retention

@@ -90,6 +90,7 @@ SIGNATURES = [
     ("psg_solve_trace", C.c_int, [C.c_void_p, _i64p, C.c_int64, C.POINTER(C.c_int64)]),
     ("psg_last_timings", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("psg_factor_info", C.c_int, [C.c_void_p, _f64p]),
+    ("psg_time_tail", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     ("psg_dense_cholinv", C.c_int, [C.c_void_p, C.c_int64, _f64p]),
     ("psg_tree_setup", C.c_int, [C.c_void_p, C.c_int32, _i32p, _i32p, _i64p, C.c_int32, _opt(_i32p), _opt(_i64p),
                                  _opt(_i32p), _i64p, _opt(_i32p), _opt(_f64p)]),

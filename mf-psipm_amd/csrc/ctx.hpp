@@ -152,6 +152,8 @@ void factor_apply2(psg_ctx* ctx, const double* r, double* z);
 // launch-shape knob of the factor's solve (2 grid, 3 poll backoff cap,
 // 4 tickets per atomic); nullptr without a factor
 int* factor_knob(psg_ctx* ctx, int knob);
+// device ms of the dense tail's product alone (-1 without a tail)
+double factor_time_tail(psg_ctx* ctx, int reps);
 // nonzero if a dependency wait of the last single-launch solve timed out
 int factor_solve_error(psg_ctx* ctx);
 // tree.hip: the network tree preconditioner

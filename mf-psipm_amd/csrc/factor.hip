@@ -2534,6 +2534,32 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   }
 }
 
+// Device time (ms, HIP events on the library stream) of the dense tail's
+// product alone, averaged over reps launches on the current solve vectors;
+// -1 without a dense tail.  The vectors are left dirty: the next solve
+// clears them, and the captured PCG iterations are captured again.
+double factor_time_tail(psg_ctx* c, int reps) {
+  FactorState* fs = c->fs;
+  if (!fs || !fs->ready || !fs->tail_nchunk || c->precond != 0) return -1.0;
+  hipStream_t s = c->stream;
+  auto run = [&]() {
+    if (c->precision == PSG_MIXED)
+      tail_solve<float, 1>(c, fs->tail_L32.p, fs->tail_Q32.p, fs->vb32.p, fs->vy32.p, fs->vx32.p, true);
+    else
+      tail_solve<double, 1>(c, fs->tail_L64.p, fs->tail_Q64.p, fs->vb64.p, fs->vy64.p, fs->vx64.p, true);
+  };
+  run();
+  HIP_CHECK(hipEventRecord(c->ev0, s));
+  for (int k = 0; k < reps; ++k) run();
+  HIP_CHECK(hipEventRecord(c->ev1, s));
+  HIP_CHECK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  fs->sf_clean = 0;
+  c->gen++;
+  return (double)ms / reps;
+}
+
 int* factor_knob(psg_ctx* c, int knob) {
   FactorState* fs = c->fs;
   if (!fs || !fs->ready) return nullptr;
@@ -2679,6 +2705,14 @@ extern "C" int psg_dense_cholinv(psg_ctx* c, int64_t n, double* a) {
     HIP_CHECK(hipMemcpyAsync(&fail, tmp.fail.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     return fail == INT_MAX ? 0 : fail + 1;
+  });
+}
+
+extern "C" int psg_time_tail(psg_ctx* c, int32_t reps, double* ms) {
+  return guarded([&]() -> int {
+    if (!c || !ms || reps < 1) throw Error(PSG_ERR_ARGUMENT, "psg_time_tail: invalid arguments");
+    *ms = factor_time_tail(c, reps);
+    return PSG_OK;
   });
 }
 
