@@ -407,6 +407,23 @@ k_update(const int* __restrict__ tile_s, const int* __restrict__ tile_i, const i
 // ---------------------------------------------------------------------------
 // triangular solves
 // ---------------------------------------------------------------------------
+// The backward solve's "not yet solved" mark: a signalling NaN with a fixed
+// payload.  Arithmetic only ever yields quiet NaNs, so no computed entry of
+// x carries it.  Every entry of x below the dense tail holds it between
+// solves; a supernode's finisher overwrites its entries once, and the
+// blocks that need them poll the entries themselves -- the datum is its own
+// ready flag, so no separate flag store (and no wait for the data store to
+// complete before it) sits on the dependency chain.
+template <class T> struct Unsolved;
+template <> struct Unsolved<double> {
+  typedef unsigned long long U;
+  static constexpr U bits = 0xFFF55A5A5A5A5A5AULL;
+};
+template <> struct Unsolved<float> {
+  typedef unsigned int U;
+  static constexpr U bits = 0xFFA5A5A5u;
+};
+
 // Solve vectors hold NR right-hand sides interleaved: entry (k, q) at k NR + q.
 template <class T, int NR = 1>
 __global__ void k_perm_in(int m, const int* __restrict__ perm, const double* __restrict__ dsc,
@@ -473,6 +490,7 @@ __global__ void k_perm_out_reset(int m, const int* __restrict__ perm, const doub
                                  int err_index, T* __restrict__ acc, T* __restrict__ y, int tail_lo,
                                  int tail_hi) {
   const int64_t nmax = max((int64_t)m, (int64_t)nflags);
+  const T mark = __builtin_bit_cast(T, Unsolved<T>::bits);
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nmax; k += (int64_t)gridDim.x * blockDim.x) {
     if (k < m) {
       const int i = perm[k];
@@ -482,10 +500,8 @@ __global__ void k_perm_out_reset(int m, const int* __restrict__ perm, const doub
       for (int q = 0; q < NR; ++q) {
         z[(int64_t)i * NR + q] = d * (double)x[k * NR + q];
         acc[k * NR + q] = T(0);
-        if (tail) {
-          x[k * NR + q] = T(0);
-          y[k * NR + q] = T(0);
-        }
+        x[k * NR + q] = tail ? T(0) : mark;
+        if (tail) y[k * NR + q] = T(0);
       }
     }
     if (k < nflags && k != err_index) flags[k] = 0;  // a timed-out wait stays reported
@@ -620,7 +636,6 @@ struct SolveArgs {
   int* cnt;  // forward arrivals per supernode
   int* ydone;
   int* pcnt; // backward arrivals per supernode
-  int* xdone;
   int* ticket;  // [0] forward, [1] backward
   int* err;
   T* acc;       // backward: per-column sums of L_b' x (length m, zeroed)
@@ -658,6 +673,34 @@ __device__ inline T load_agent(const T* p) {
 template <class T>
 __device__ inline void store_agent(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Poll one entry of x until its finisher has stored it; bounded like
+// wait_at_least (a timed-out poll raises *err and returns the mark).
+template <class T>
+__device__ inline T wait_solved(const T* p, int* err, int sleep_max) {
+  typedef typename Unsolved<T>::U U;
+  U* q = reinterpret_cast<U*>(const_cast<T*>(p));
+  U v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int nap = 1;
+  for (long it = 0; v == Unsolved<T>::bits; ++it) {
+    if (it == kSpinLimit) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    for (int k = 0; k < nap; ++k) __builtin_amdgcn_s_sleep(2);
+    if (nap < sleep_max) nap <<= 1;
+    v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return __builtin_bit_cast(T, v);
+}
+
+template <class T>
+__global__ void k_mark_unsolved(T* __restrict__ p, int64_t n) {
+  const T mark = __builtin_bit_cast(T, Unsolved<T>::bits);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = mark;
 }
 
 // stage the w x w block inverse into the full 64 x 64 LDS tile (zero
@@ -833,7 +876,6 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
   __shared__ T xr[NR][kBlock];
   __shared__ T cs[NR][kW];
   __shared__ T part_sh[NR][kBlock / kWave][kWave];
-  __shared__ int tg[kBlock];
   __shared__ int sh_u;
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1), wv = tid / kWave;
@@ -862,8 +904,9 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
     // and count themselves in pcnt, and take their tickets before it, so the
     // finisher only waits on smaller tickets.  It adds its own sums from LDS
     // to acc -- no atomics, no arrival count on the critical path -- and
-    // its block inverse and y_s, which do not depend on x, are fetched
-    // before the wait.  A single-unit supernode is its own finisher.
+    // its block inverse, y_s and the other blocks' sums, which do not depend
+    // on the parent, are fetched while the parent is still being solved.  A
+    // single-unit supernode is its own finisher.
     const int total = a.su_range[2 * s + 1] - a.su_range[2 * s];
     const bool single = u == a.su_range[2 * s];  // the finisher
     T ys[NR];
@@ -872,12 +915,8 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
 #pragma unroll
       for (int q = 0; q < NR; ++q) ys[q] = tid < w ? a.y[(int64_t)(f + tid) * NR + q] : T(0);
     }
-    // wait for the ancestors owning this block's rows
     int row = -1;
-    if (tid < nrow) {
-      row = a.rows[a.rptr[s] + r0 + tid];
-      tg[tid] = a.snode_of[row];
-    }
+    if (tid < nrow) row = a.rows[a.rptr[s] + r0 + tid];
     // panel block of this wave (rows [64 wv, 64 wv + 64) of the unit),
     // fetched before waiting on x: lane (rq, cg) = (lane & 15, lane >> 4)
     // holds rows rq + 16k and columns cg + 4j, so every load instruction
@@ -895,19 +934,22 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
           v[k][j] = (c < w && i0 + rq + 16 * k < nrow) ? P[(int64_t)c * nr + 16 * k] : T(0);
         }
     }
-    __syncthreads();
-    if (tid < nrow && (tid == 0 || tg[tid - 1] != tg[tid])) wait_at_least(&a.xdone[tg[tid]], 1, a.err, a.sleep_max);
-    // the finisher also waits for the supernode's other blocks
-    if (single && total > 1 && tid == kBlock - 1) wait_at_least(&a.pcnt[s], total - 1, a.err, a.sleep_max);
-    __syncthreads();
-    if (a.trace && tid == 0) a.trace[6 * (int64_t)u + 4] = wall_clock64();
-#pragma unroll
-    for (int q = 0; q < NR; ++q) xr[q][tid] = (tid < nrow) ? load_agent(a.x + (int64_t)row * NR + q) : T(0);
+    // the other blocks' column sums: counted in pcnt before they are read
     T others[NR];
 #pragma unroll
+    for (int q = 0; q < NR; ++q) others[q] = T(0);
+    if (single && total > 1 && tid < w) {
+      wait_at_least(&a.pcnt[s], total - 1, a.err, a.sleep_max);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) others[q] = load_agent(a.acc + (int64_t)(f + tid) * NR + q);
+    }
+    // the block's rows of x, polled until their finishers (or the dense
+    // tail, final before this launch) have stored them
+#pragma unroll
     for (int q = 0; q < NR; ++q)
-      others[q] = (single && total > 1 && tid < w) ? load_agent(a.acc + (int64_t)(f + tid) * NR + q) : T(0);
+      xr[q][tid] = (tid < nrow) ? wait_solved(a.x + (int64_t)row * NR + q, a.err, a.sleep_max) : T(0);
     __syncthreads();
+    if (a.trace && tid == 0) a.trace[6 * (int64_t)u + 4] = wall_clock64();
     // partial[c] = sum over the wave's rows of L[r][c] x[row]; the panel
     // block in registers serves every right-hand side
 #pragma unroll
@@ -925,28 +967,35 @@ __global__ void __launch_bounds__(kBlock) k_bwd_sf(SolveArgs<T> a) {
     }
     __syncthreads();
     if (single) {
-      if (tid < w) {
+      if (tid < kW) {
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
           const T sum = nrow > 0 ? (part_sh[q][0][tid] + part_sh[q][1][tid]) + (part_sh[q][2][tid] + part_sh[q][3][tid])
                                  : T(0);
-          cs[q][tid] = ys[q] - (others[q] + sum);
+          cs[q][tid] = tid < w ? ys[q] - (others[q] + sum) : T(0);
         }
+      }
+      __syncthreads();
+      // x_s = L_ss^-T cs: wave wv sums rows [16 wv, 16 wv + 16) of the
+      // inverse (zero above its diagonal and beyond w) for column `lane`
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        T acc = T(0);
+#pragma unroll
+        for (int k = 0; k < kW / 4; ++k) {
+          const int j = wv * (kW / 4) + k;
+          acc += sD[j][lane] * cs[q][j];
+        }
+        part_sh[q][wv][lane] = acc;
       }
       __syncthreads();
       if (tid < w) {
 #pragma unroll
-        for (int q = 0; q < NR; ++q) {
-          T acc = T(0);
-          for (int j = tid; j < w; ++j) acc += sD[j][tid] * cs[q][j];
-          store_agent(a.x + (int64_t)(f + tid) * NR + q, acc);
-        }
+        for (int q = 0; q < NR; ++q)
+          store_agent(a.x + (int64_t)(f + tid) * NR + q,
+                      (part_sh[q][0][tid] + part_sh[q][1][tid]) + (part_sh[q][2][tid] + part_sh[q][3][tid]));
       }
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
-      if (tid == 0) __hip_atomic_store(&a.xdone[s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (a.trace && tid == 0) a.trace[6 * (int64_t)u + 5] = wall_clock64();
-      __syncthreads();
       continue;
     }
     if (tid < w && nrow > 0) {
@@ -1632,10 +1681,7 @@ __device__ inline void symv_tile(int r, int ld, int sc, int J, int I, const T* _
 template <class T, int NR, int KV>
 __global__ void __launch_bounds__(kBlock)
 k_tail_symv(int r, int ld, int sc, const int* __restrict__ tiles, const T* __restrict__ Q, const T* __restrict__ b,
-            T* __restrict__ x, int* __restrict__ done, int ndone) {
-  // the tail supernodes count as solved for the backward launch that follows
-  if (blockIdx.x == 0)
-    for (int q = threadIdx.x; q < ndone; q += blockDim.x) done[q] = 1;
+            T* __restrict__ x) {
   auto bload = [&](int k, int e) { return (double)b[(int64_t)k * NR + e]; };
   symv_tile<T, NR, KV>(r, ld, sc, tiles[2 * blockIdx.x], tiles[2 * blockIdx.x + 1], Q, bload, x);
 }
@@ -1958,14 +2004,12 @@ static void tail_solve(psg_ctx* c, const T* X, const T* Q, T* vb, T* vy, T* vx, 
     const int ld = fs->tail_ld, sc = fs->tail_sc;
     const int* tl = fs->tail_sym_tiles.p;
     const dim3 grid(fs->tail_nsym), blk(kBlock);
-    int* done = fs->flags.p + 3 * fs->nsuper + fs->tail_s0;
-    const int nd = fs->tail_nchunk;
     if (fs->tail_kv == 4)
-      hipLaunchKernelGGL((k_tail_symv<T, NR, 4>), grid, blk, 0, st, r, ld, sc, tl, Q, bt, xt, done, nd);
+      hipLaunchKernelGGL((k_tail_symv<T, NR, 4>), grid, blk, 0, st, r, ld, sc, tl, Q, bt, xt);
     else if (fs->tail_kv == 2)
-      hipLaunchKernelGGL((k_tail_symv<T, NR, 2>), grid, blk, 0, st, r, ld, sc, tl, Q, bt, xt, done, nd);
+      hipLaunchKernelGGL((k_tail_symv<T, NR, 2>), grid, blk, 0, st, r, ld, sc, tl, Q, bt, xt);
     else
-      hipLaunchKernelGGL((k_tail_symv<T, NR, 1>), grid, blk, 0, st, r, ld, sc, tl, Q, bt, xt, done, nd);
+      hipLaunchKernelGGL((k_tail_symv<T, NR, 1>), grid, blk, 0, st, r, ld, sc, tl, Q, bt, xt);
     HIP_LAUNCH_CHECK();
     return;
   }
@@ -1983,10 +2027,6 @@ static void tail_solve(psg_ctx* c, const T* X, const T* Q, T* vb, T* vy, T* vx, 
   hipLaunchKernelGGL((k_tail_upper<T, NR>), dim3(fs->tail_nupper), dim3(kBlock), 0, st, r, fs->tail_ur,
                      fs->tail_upper_tiles.p, X,
                      vy + (size_t)F * NR, vx + (size_t)F * NR);
-  HIP_LAUNCH_CHECK();
-  // tail supernodes count as solved for the backward dependency waits
-  hipLaunchKernelGGL(k_fill_int, dim3(grid_for(fs->tail_nchunk, kBlock, 1024)), dim3(kBlock), 0, st,
-                     fs->flags.p + 3 * fs->nsuper + fs->tail_s0, fs->tail_nchunk, 1);
   HIP_LAUNCH_CHECK();
 }
 
@@ -2458,7 +2498,13 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   // counters, column sums and tail targets are left cleared by the previous
   // solve's last kernel; cleared here only after a new factor or a failure
   const bool clean = fs->sf_clean == NR;
-  if (!clean) zero_async(fs->flags.p, (int64_t)fs->flags.n, s);
+  if (!clean) {
+    zero_async(fs->flags.p, (int64_t)fs->flags.n, s);
+    // x starts unsolved; the dense tail's part is zeroed by tail_solve
+    hipLaunchKernelGGL((k_mark_unsolved<T>), dim3(grid_for((int64_t)m * NR, kBlock, 2048)), dim3(kBlock), 0, s, vx,
+                       (int64_t)m * NR);
+    HIP_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL((k_perm_in_iso<T, NR>), dim3(grid_for(std::max(m, fs->n_iso), kBlock, 4096)), dim3(kBlock), 0, s,
                      m, fs->perm.p, c->dsc.p, r, vb, fs->n_iso, fs->iso_col.p, fs->iso_doff.p, dinv, vy, vx);
   HIP_LAUNCH_CHECK();
@@ -2485,7 +2531,6 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   a.cnt = fs->flags.p;
   a.ydone = fs->flags.p + ns;
   a.pcnt = fs->flags.p + 2 * ns;
-  a.xdone = fs->flags.p + 3 * ns;
   a.ticket = fs->flags.p + 4 * ns;
   a.err = fs->flags.p + 4 * ns + 4;
   a.acc = vb + (size_t)m * NR;  // second half of the solve buffer, zeroed below
