@@ -2063,7 +2063,7 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
     // chain of the single-launch solve by two bandwidth-bound products
     const int tail_min = (int)knobd("PSG_TAIL_MIN", 128);
     const int tail_max = (int)knobd("PSG_TAIL_MAX", 24576);
-    const double rho = knobd("PSG_TAIL_DENSITY", 0.8);
+    const double rho = knobd("PSG_TAIL_DENSITY", 0.9);
     // below this dense size (entries) a suffix is inverted whatever its
     // density: streaming a few MB beats the latency of the dependency chain
     const double small = knobd("PSG_TAIL_SMALL", 4.0e6);
