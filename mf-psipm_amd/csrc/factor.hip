@@ -190,8 +190,24 @@ __device__ inline bool lds_potrf(T (*sA)[kW + 1], int w) {
     if (tid == 0) sA[j][j] = ljj;
     __syncthreads();
     if (row > j && row < w) {
+      // every load of the update is issued before any store (the stores
+      // could alias the loads, which would otherwise serialise them)
+      constexpr int G = kBlock / kW, NK = kW / G;
       const T lij = sA[row][j];
-      for (int k = j + 1 + grp; k <= row; k += kBlock / kW) sA[row][k] -= lij * sA[k][j];
+      T lk[NK], ak[NK];
+#pragma unroll
+      for (int t = 0; t < NK; ++t) {
+        const int k = j + 1 + grp + G * t;
+        if (k <= row) {
+          lk[t] = sA[k][j];
+          ak[t] = sA[row][k];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < NK; ++t) {
+        const int k = j + 1 + grp + G * t;
+        if (k <= row) sA[row][k] = ak[t] - lij * lk[t];
+      }
     }
   }
   __syncthreads();
@@ -1330,25 +1346,33 @@ k_mfma_gemm(MGemm single, int ntm, const MGemm* __restrict__ descs, const int* _
       }
 }
 
-// Dense tail, 64-column step j: factor the w x w diagonal block in LDS and
-// leave its inverse in place (the factor itself is not needed afterwards:
-// the panel solve and the inversion use the inverse).
+// Dense tail, 64-column step j: factor the w x w diagonal block in LDS.  Its
+// inverse, which the tail inversion starts from, is formed for all diagonal
+// blocks at once after the factorisation (k_dense_diag_inv): inverting
+// inside each step would put 64 more dependent LDS sweeps per step on the
+// factorisation's sequential chain.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
 k_dense_potrf(T* __restrict__ L, int r, int j, int w, int F, int* __restrict__ fail) {
   __shared__ T sA[kW][kW + 1];
-  __shared__ T tmp[kW];
   T* Pj = L + (int64_t)j * r + j;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < w * w; idx += blockDim.x) {
-    const int c = idx / w, rr = idx % w;
-    sA[rr][c] = (rr >= c) ? Pj[(int64_t)c * r + rr] : T(0);
+  {
+    T v[kW * kW / kBlock];  // all loads in flight before the LDS stores
+#pragma unroll
+    for (int k = 0; k < kW * kW / kBlock; ++k) {
+      const int idx = tid + k * kBlock, c = idx / kW, rr = idx % kW;
+      v[k] = (c < w && rr < w && rr >= c) ? Pj[(int64_t)c * r + rr] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < kW * kW / kBlock; ++k) {
+      const int idx = tid + k * kBlock;
+      sA[idx % kW][idx / kW] = v[k];
+    }
   }
   __syncthreads();
   const bool bad = lds_potrf(sA, w);
   if (bad && tid == 0) atomicMin(fail, F + j);
-  __syncthreads();
-  lds_trinv(sA, tmp, w);
   __syncthreads();
   for (int idx = tid; idx < w * w; idx += blockDim.x) {
     const int c = idx / w, rr = idx % w;
@@ -1356,18 +1380,20 @@ k_dense_potrf(T* __restrict__ L, int r, int j, int w, int F, int* __restrict__ f
   }
 }
 
-// Dense tail, rows below step j: L[i, j:j+w] <- L[i, j:j+w] L_jj^-T, one
-// row per lane, in registers (entry c needs the original entries 0..c, so
-// the entries are replaced from the last down).
+// Dense tail, rows below step j: L[i, j:j+w] <- L[i, j:j+w] L_jj^-T, one row
+// per lane, in registers: the forward substitution a[c] = (a[c] - sum_{k<c}
+// a[k] L_jj[c][k]) / L_jj[c][c], with four partial sums per entry.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
 k_dense_trsm(T* __restrict__ L, int r, int j, int w) {
   __shared__ T sD[kW][kW + 1];
+  __shared__ T sR[kW];
   const T* D = L + (int64_t)j * r + j;
   for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
     const int k = idx / w, c = idx % w;
     sD[c][k] = (c >= k) ? D[(int64_t)k * r + c] : T(0);
   }
+  if (threadIdx.x < w) sR[threadIdx.x] = T(1) / D[(int64_t)threadIdx.x * r + threadIdx.x];
   __syncthreads();
   const int64_t i = (int64_t)j + w + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= r) return;
@@ -1375,14 +1401,55 @@ k_dense_trsm(T* __restrict__ L, int r, int j, int w) {
 #pragma unroll
   for (int k = 0; k < kW; ++k) a[k] = (k < w) ? L[(int64_t)(j + k) * r + i] : T(0);
 #pragma unroll
-  for (int c = kW - 1; c >= 0; --c) {
+  for (int c = 0; c < kW; ++c) {
     if (c < w) {
-      T acc = T(0);
+      T s0 = a[c], s1 = T(0), s2 = T(0), s3 = T(0);
 #pragma unroll
-      for (int k = 0; k <= c; ++k) acc += a[k] * sD[c][k];
-      a[c] = acc;
-      L[(int64_t)(j + c) * r + i] = acc;
+      for (int k = 0; k + 3 < c; k += 4) {
+        s0 -= a[k] * sD[c][k];
+        s1 -= a[k + 1] * sD[c][k + 1];
+        s2 -= a[k + 2] * sD[c][k + 2];
+        s3 -= a[k + 3] * sD[c][k + 3];
+      }
+#pragma unroll
+      for (int k = c & ~3; k < c; ++k) s0 -= a[k] * sD[c][k];
+      a[c] = ((s0 + s1) + (s2 + s3)) * sR[c];
+      L[(int64_t)(j + c) * r + i] = a[c];
     }
+  }
+}
+
+// Dense tail after its factorisation: every 64-column diagonal block
+// replaced by its inverse, one block per workgroup, all in parallel (the
+// base case of the recursive inversion X = L_T^-1).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_dense_diag_inv(T* __restrict__ L, int r) {
+  __shared__ T sA[kW][kW + 1];
+  __shared__ T tmp[kW];
+  const int j = blockIdx.x * kW;
+  const int w = min(kW, r - j);
+  T* Pj = L + (int64_t)j * r + j;
+  const int tid = threadIdx.x;
+  {
+    T v[kW * kW / kBlock];
+#pragma unroll
+    for (int k = 0; k < kW * kW / kBlock; ++k) {
+      const int idx = tid + k * kBlock, c = idx / kW, rr = idx % kW;
+      v[k] = (c < w && rr < w && rr >= c) ? Pj[(int64_t)c * r + rr] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < kW * kW / kBlock; ++k) {
+      const int idx = tid + k * kBlock;
+      sA[idx % kW][idx / kW] = v[k];
+    }
+  }
+  __syncthreads();
+  lds_trinv(sA, tmp, w);
+  __syncthreads();
+  for (int idx = tid; idx < w * w; idx += blockDim.x) {
+    const int c = idx / w, rr = idx % w;
+    if (rr >= c) Pj[(int64_t)c * r + rr] = sA[rr][c];
   }
 }
 
@@ -1984,6 +2051,8 @@ static void dense_factor_inverse(FactorState* fs, hipStream_t st, int r, int F, 
     }
     HIP_LAUNCH_CHECK();
   }
+  hipLaunchKernelGGL((k_dense_diag_inv<T>), dim3((r + kW - 1) / kW), dim3(kBlock), 0, st, L, r);
+  HIP_LAUNCH_CHECK();
   for (const auto& g : fs->inv_launches) {
     if (!g.ntile) continue;
     hipLaunchKernelGGL((k_mfma_gemm<T>), dim3(g.ntile), dim3(kBlock), 0, st, MGemm{}, 1, fs->inv_descs.p,
