@@ -650,7 +650,6 @@ struct SolveArgs {
   T* x;      // backward result
   T* part;   // backward partials, kW per unit
   int* cnt;  // forward arrivals per supernode
-  int* ydone;
   int* pcnt; // backward arrivals per supernode
   int* ticket;  // [0] forward, [1] backward
   int* err;
@@ -1802,10 +1801,6 @@ k_dense_apply(int r, int F, int ld, int sc, const int* __restrict__ tiles, const
   if (threadIdx.x == 0) __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ void k_fill_int(int* __restrict__ p, int n, int v) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
-}
-
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
@@ -2339,6 +2334,8 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
     fs->su_n.upload(un, s);
     fs->su_range.upload(range, s);
     fs->need.upload(need, s);
+    // flags: [0, ns) forward arrivals, [2 ns, 3 ns) backward arrivals, 4 ns +
+    // {0, 1} tickets, 4 ns + 4 the timed-out-wait flag (the other ranges unused)
     fs->flags.alloc(4 * (size_t)P.nsuper + 8);
     zero_async(fs->flags.p, (int64_t)fs->flags.n, s);
     if (c->precision == PSG_MIXED) {
@@ -2598,7 +2595,6 @@ static void apply_run_sf(psg_ctx* c, const double* r, double* z, const T* panels
   a.x = vx;
   a.part = part;
   a.cnt = fs->flags.p;
-  a.ydone = fs->flags.p + ns;
   a.pcnt = fs->flags.p + 2 * ns;
   a.ticket = fs->flags.p + 4 * ns;
   a.err = fs->flags.p + 4 * ns + 4;
