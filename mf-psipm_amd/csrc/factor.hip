@@ -97,14 +97,17 @@ struct FactorState {
   int tail_tc = 512, tail_ur = 1024;  // lower-tile columns, upper-tile rows
   DevBuf<double> tail_L64, tail_S64;
   DevBuf<float> tail_L32, tail_S32;
-  // symmetric tail (tail_sym = 1, the FP64 default): Q = X' X in tail_Q
-  // (leading dimension tail_ld), applied by one product over tail_sym_tiles
-  // of tail_sc columns x (kBlock V tail_kv) rows; 0 keeps the two triangular
-  // products with X
+  // symmetric tail: Q = X' X in tail_Q (leading dimension tail_ld), applied
+  // by one product over tail_sym_tiles of tail_sc columns x (kBlock V
+  // tail_kv) rows.  tail_sym 1: Q in the factor's precision (the FP64
+  // default); 2: an FP64 Q with FP32 solve vectors (the mixed default);
+  // 0: the two triangular products with X
   int tail_sym = 1, tail_ld = 0, tail_sc = kSC, tail_kv = 4, tail_nsym = 0;
   DevBuf<int> tail_sym_tiles;
   DevBuf<double> tail_Q64;
   DevBuf<float> tail_Q32;
+  DevBuf<double> tail_X64;    // mode 2 (mixed precision): X widened to FP64
+  DevBuf<double> tail_acc64;  // mode 2: FP64 sums of the tail product (zero between solves)
   // the tail spans every row (small LPs): k_dense_apply does the whole apply,
   // accumulating in dense_acc (cleared by its last workgroup)
   bool dense_all = false;
@@ -1631,9 +1634,9 @@ k_tail_upper_vec(int r, int ur, const int* __restrict__ tiles, const T* __restri
 // the wave, then over the four waves in LDS); the diagonal adds once.
 // One tile of the symmetric product; bload(k, e) is entry (k, e) of the
 // right-hand side (NR interleaved columns).
-template <class T, int NR, int KV, class BL>
+template <class T, int NR, int KV, class BL, class TX = T>
 __device__ inline void symv_tile(int r, int ld, int sc, int J, int I, const T* __restrict__ Q, const BL& bload,
-                                 T* __restrict__ x) {
+                                 TX* __restrict__ x) {
   typedef typename Vec16<T>::v VT;
   constexpr int V = Vec16<T>::n;
   constexpr int RW = kWave * V * KV;         // rows per wave
@@ -1728,7 +1731,7 @@ __device__ inline void symv_tile(int r, int ld, int sc, int J, int I, const T* _
     double sm = 0.0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) sm += cpart[w][e][cc];
-    __hip_atomic_fetch_add(x + (int64_t)(j0 + cc) * NR + e, (T)sm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(x + (int64_t)(j0 + cc) * NR + e, (TX)sm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 #pragma unroll
   for (int k = 0; k < KV; ++k)
@@ -1738,18 +1741,33 @@ __device__ inline void symv_tile(int r, int ld, int sc, int J, int I, const T* _
       if (row < r && row >= j0) {
 #pragma unroll
         for (int e = 0; e < NR; ++e)
-          __hip_atomic_fetch_add(x + (int64_t)row * NR + e, (T)racc[k][t][e], __ATOMIC_RELAXED,
+          __hip_atomic_fetch_add(x + (int64_t)row * NR + e, (TX)racc[k][t][e], __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_AGENT);
       }
     }
 }
 
-template <class T, int NR, int KV>
+// Q in T, the right-hand side in TB, the accumulation target in TX (mode 2:
+// FP64 Q, FP32 b, an FP64 accumulator rounded once by k_tail_round)
+template <class T, int NR, int KV, class TB = T, class TX = T>
 __global__ void __launch_bounds__(kBlock)
-k_tail_symv(int r, int ld, int sc, const int* __restrict__ tiles, const T* __restrict__ Q, const T* __restrict__ b,
-            T* __restrict__ x) {
+k_tail_symv(int r, int ld, int sc, const int* __restrict__ tiles, const T* __restrict__ Q, const TB* __restrict__ b,
+            TX* __restrict__ x) {
   auto bload = [&](int k, int e) { return (double)b[(int64_t)k * NR + e]; };
-  symv_tile<T, NR, KV>(r, ld, sc, tiles[2 * blockIdx.x], tiles[2 * blockIdx.x + 1], Q, bload, x);
+  symv_tile<T, NR, KV, decltype(bload), TX>(r, ld, sc, tiles[2 * blockIdx.x], tiles[2 * blockIdx.x + 1], Q, bload, x);
+}
+
+// x_T = the FP64 sums rounded once to T, and the sums cleared for the next
+// solve.  Rounding every tile's partial sum into a T accumulator instead
+// would leave errors of u |Q| |b| (cancellation across ~r/64 tiles), enough
+// to make x'b negative once cond(Q) nears 1/u; one final rounding keeps them
+// at u |Q b|.
+template <class T>
+__global__ void k_tail_round(int64_t n, double* __restrict__ acc, T* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] = (T)acc[i];
+    acc[i] = 0.0;
+  }
 }
 
 // The whole apply of a factor that is one dense block plus isolated 1x1
@@ -1846,10 +1864,15 @@ static void setup_tail(psg_ctx* c, const Plan& P) {
   // cond(L_T L_T') stays well below 1 / (unit roundoff): always for FP64 at
   // the shifts the ladder uses, but not for FP32, where x = X' (X b) keeps
   // b' x = |X b|^2 >= 0 by construction.  PSG_TAIL_SYM=0/1 overrides.
+  // Mixed precision applies the tail with an FP64 Q formed from the FP32
+  // inverse (mode 2): the same bytes as the two FP32 triangular products,
+  // one launch, and Q's rounding stays far below its smallest eigenvalue.
   const char* sym = getenv("PSG_TAIL_SYM");
-  fs->tail_sym = sym ? (sym[0] == '0' ? 0 : 1) : (c->precision == PSG_MIXED ? 0 : 1);
+  if (sym) fs->tail_sym = sym[0] == '0' ? 0 : (sym[0] == '2' && c->precision == PSG_MIXED ? 2 : 1);
+  else fs->tail_sym = c->precision == PSG_MIXED ? 2 : 1;
   if (fs->tail_sym) {
-    const int V = c->precision == PSG_MIXED ? Vec16<float>::n : Vec16<double>::n;
+    const bool q32 = c->precision == PSG_MIXED && fs->tail_sym == 1;
+    const int V = q32 ? Vec16<float>::n : Vec16<double>::n;
     fs->tail_ld = (r + 3) / 4 * 4;
     // one vector per lane (512-row FP64 tiles) measured fastest: more
     // resident waves beat more loads per wave; PSG_TAIL_KV = 2 or 4 widens
@@ -1878,12 +1901,20 @@ static void setup_tail(psg_ctx* c, const Plan& P) {
     fs->tail_nsym = (int)qt.size() / 2;
     fs->tail_sym_tiles.upload(qt, st);
     const size_t nq = (size_t)fs->tail_ld * r;
-    if (c->precision == PSG_MIXED) {
+    if (q32) {
       fs->tail_Q32.alloc(nq);
       fs->tail_Q64.release();
     } else {
       fs->tail_Q64.alloc(nq);
       fs->tail_Q32.release();
+    }
+    if (fs->tail_sym == 2) {
+      fs->tail_X64.alloc((size_t)r * r);  // the FP32 inverse widened for Q's product
+      fs->tail_acc64.alloc((size_t)r * kMaxRhs);
+      zero_async(fs->tail_acc64.p, (int64_t)r * kMaxRhs, st);
+    } else {
+      fs->tail_X64.release();
+      fs->tail_acc64.release();
     }
   } else {
     fs->tail_Q64.release();
@@ -2001,6 +2032,12 @@ static void dense_factor_inverse(FactorState* fs, hipStream_t st, int r, int F, 
 // rest of the tree.  Gather them into L (r x r), factor it, and overwrite it
 // with X = L^-1.
 template <class T>
+__global__ void k_widen(int64_t n, const T* __restrict__ in, double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (double)in[i];
+}
+
+template <class T>
 static void tail_factor_dense(psg_ctx* c, const T* panels, T* L, T* Sc, T* Q) {
   FactorState* fs = c->fs;
   hipStream_t st = c->stream;
@@ -2022,8 +2059,17 @@ static void tail_factor_dense(psg_ctx* c, const T* panels, T* L, T* Sc, T* Q) {
     g.lower = 1; g.diag = 0; g.tri = 3;
     g.alpha = 1.0; g.beta = 0.0;
     const int ntm = (r + kMT - 1) / kMT;
-    hipLaunchKernelGGL((k_mfma_gemm<T>), dim3(mgemm_blocks(g)), dim3(kBlock), 0, st, g, ntm, (const MGemm*)nullptr,
-                       (const int*)nullptr, (const int*)nullptr, L, Q);
+    if (fs->tail_sym == 2) {
+      // mixed precision: Q = X' X in FP64 from the FP32 inverse
+      double* X64 = fs->tail_X64.p;
+      hipLaunchKernelGGL((k_widen<T>), dim3(grid_for(rr, kBlock, 16384)), dim3(kBlock), 0, st, rr, L, X64);
+      HIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL((k_mfma_gemm<double>), dim3(mgemm_blocks(g)), dim3(kBlock), 0, st, g, ntm,
+                         (const MGemm*)nullptr, (const int*)nullptr, (const int*)nullptr, X64, fs->tail_Q64.p);
+    } else {
+      hipLaunchKernelGGL((k_mfma_gemm<T>), dim3(mgemm_blocks(g)), dim3(kBlock), 0, st, g, ntm,
+                         (const MGemm*)nullptr, (const int*)nullptr, (const int*)nullptr, L, Q);
+    }
     HIP_LAUNCH_CHECK();
   }
 }
@@ -2068,7 +2114,20 @@ static void tail_solve(psg_ctx* c, const T* X, const T* Q, T* vb, T* vy, T* vx, 
     const int ld = fs->tail_ld, sc = fs->tail_sc;
     const int* tl = fs->tail_sym_tiles.p;
     const dim3 grid(fs->tail_nsym), blk(kBlock);
-    if (fs->tail_kv == 4)
+    if (fs->tail_sym == 2) {  // FP64 Q and sums, vectors in T
+      const double* Q64 = fs->tail_Q64.p;
+      double* acc = fs->tail_acc64.p;
+      if (!clean) zero_async(acc, (int64_t)r * NR, st);
+      if (fs->tail_kv == 4)
+        hipLaunchKernelGGL((k_tail_symv<double, NR, 4, T, double>), grid, blk, 0, st, r, ld, sc, tl, Q64, bt, acc);
+      else if (fs->tail_kv == 2)
+        hipLaunchKernelGGL((k_tail_symv<double, NR, 2, T, double>), grid, blk, 0, st, r, ld, sc, tl, Q64, bt, acc);
+      else
+        hipLaunchKernelGGL((k_tail_symv<double, NR, 1, T, double>), grid, blk, 0, st, r, ld, sc, tl, Q64, bt, acc);
+      HIP_LAUNCH_CHECK();
+      hipLaunchKernelGGL((k_tail_round<T>), dim3(grid_for((int64_t)r * NR, kBlock, 1024)), dim3(kBlock), 0, st,
+                         (int64_t)r * NR, acc, xt);
+    } else if (fs->tail_kv == 4)
       hipLaunchKernelGGL((k_tail_symv<T, NR, 4>), grid, blk, 0, st, r, ld, sc, tl, Q, bt, xt);
     else if (fs->tail_kv == 2)
       hipLaunchKernelGGL((k_tail_symv<T, NR, 2>), grid, blk, 0, st, r, ld, sc, tl, Q, bt, xt);
@@ -2356,7 +2415,7 @@ static void upload_plan(psg_ctx* c, const Plan& P) {
     if (getenv("PSG_SF_TRACE")) fs->trace.alloc(6 * (size_t)fs->sf_units);
     else fs->trace.release();
     // one dense block spans the whole factor: the apply is one launch
-    fs->dense_all = fs->tail_nchunk && fs->tail_sym && fs->tail_r + fs->n_iso == P.m && fs->sf_units == 0 &&
+    fs->dense_all = fs->tail_nchunk && fs->tail_sym == 1 && fs->tail_r + fs->n_iso == P.m && fs->sf_units == 0 &&
                     fs->low_ptr.size() <= 1 && !getenv("PSG_DENSE_APPLY_OFF");
     if (fs->dense_all) {
       const size_t na = (size_t)fs->tail_r * kMaxRhs;

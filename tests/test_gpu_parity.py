@@ -278,8 +278,12 @@ def test_dense_tail_inverse_solve(gpu, monkeypatch, precision, m, n, k, frac, ta
     monkeypatch.setenv("PSG_TAIL_MIN", str(tail_min))
     outs, infos = [], []
     # level-scheduled substitution; single launch with the symmetric tail
-    # product (L_T L_T')^-1 b; single launch with the two triangular products
-    for mode, sym in (("level", "1"), ("single", "1"), ("single", "0")):
+    # product (L_T L_T')^-1 b; single launch with the two triangular products;
+    # mixed precision also with the FP64 Q over FP32 vectors (its default)
+    modes = [("level", "1"), ("single", "1"), ("single", "0")]
+    if precision == "mixed":
+        modes.append(("single", "2"))
+    for mode, sym in modes:
         monkeypatch.setenv("PSG_SOLVE", mode)
         monkeypatch.setenv("PSG_TAIL_SYM", sym)
         obj = HipSpectralStableColumnPreconditioner(A, protected=~lp.pos, precision=precision)
@@ -296,9 +300,14 @@ def test_dense_tail_inverse_solve(gpu, monkeypatch, precision, m, n, k, frac, ta
     tail_r = infos[1][3]
     assert infos[1][0] == 1 and tail_r >= tail_min and infos[1][4] >= 2
     assert infos[1][6] == 1 and infos[2][6] == 0 and infos[2][3] == tail_r
-    (l1, _), (s1, s2), (t1, t2) = outs
+    (l1, _), (s1, s2), (t1, t2) = outs[:3]
     tol = 1e-11 if precision == "fp64" else 1e-4
-    for a, b in ((s1, l1), (s2, s1), (t1, l1), (t2, t1)):
+    pairs = [(s1, l1), (s2, s1), (t1, l1), (t2, t1)]
+    if precision == "mixed":
+        assert infos[3][6] == 2 and infos[3][3] == tail_r
+        u1, u2 = outs[3]
+        pairs += [(u1, l1), (u2, u1)]
+    for a, b in pairs:
         assert np.linalg.norm(a - b) <= tol * np.linalg.norm(b)
     apply_ref, cond = _dense_precond(A, core, 1.0 / theta, 1e-2)
     ref = apply_ref(r)
