@@ -459,6 +459,22 @@ def main():
                       f"because the GPU's core takes minutes to factor on the CPU; factor "
                       f"({cpu['factor_s']:.1f}s) excluded"},
     }
+    # the headline comparisons of the metric: mixed precision (and, for time
+    # to gap, the block solve and the learned selection) against the plain
+    # FP64 GPU path on the same workloads
+    def ratio(a, b):
+        return None if a is None or b is None or not b else round(a / b, 3)
+
+    def seconds(d, key):
+        v = (d or {}).get(key)
+        return None if not v or v.get("status") != "optimal" else v["seconds"]
+
+    line["headline"] = {
+        "pcg_it_per_s_mixed_over_fp64": ratio(line.get("mixed_it_per_s"), line["value"]),
+        "time_to_gap_2000x6000_fp64_over_mixed_block": ratio(seconds(gaps, "fp64"), seconds(gaps, "mixed_block")),
+        "time_to_gap_setcover_fp64_over_learned_mixed_block": ratio(seconds(learned, "plain_fp64"),
+                                                                    seconds(learned, "learned_mixed_block")),
+    }
     print(json.dumps(line), flush=True)
 
 
