@@ -249,9 +249,10 @@ int psg_dense_cholinv(psg_ctx* ctx, int64_t n, double* a);
 /* Diagnostics of the factor's solve layout, out[8] = {solve mode (0 level
  * scheduled, 1 single launch), dependency-driven units, isolated 1x1
  * supernodes, dense-tail order r (0: none), dense-tail supernodes, nsuper,
- * dense-tail apply (1: one symmetric product with (L_T L_T')^-1, 0: two
- * triangular products with L_T^-1, or no tail), panel entries of the
- * dense-tail supernodes}.
+ * dense-tail apply (1: one symmetric product with (L_T L_T')^-1 in the
+ * factor's precision, 2: the same product with an FP64 matrix and FP64 sums
+ * over FP32 vectors, 0: two triangular products with L_T^-1, or no tail),
+ * panel entries of the dense-tail supernodes}.
  * No counterpart in the reference. */
 int psg_factor_info(psg_ctx* ctx, double* out);
 /* Device time (ms, HIP events on the context's stream) of the dense tail's
